@@ -1,0 +1,125 @@
+/*
+ * hipla.h -- C ABI of libhipla.so, the MI355X (gfx950) per-cell dense solve / inverse library.
+ *
+ * Plain C types only (pointers and sizes).  Host pointers where the reference passes host
+ * arrays; device pointers (hipMalloc'd or torch tensors' data_ptr) in the batched API.
+ */
+#ifndef HIPLA_H
+#define HIPLA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------------------------ *
+ * Drop-in entry points.  Same names, signatures, argument meaning and return codes as the    *
+ * reference (declared at source/templates.h:358-360, defined at source/matrix_cpu.c:273-349). *
+ * ------------------------------------------------------------------------------------------ */
+
+/* Replaces solve_matrix, source/matrix_cpu.c:309-321 (-> cpu_solve_matrix :96-204 or
+ * gpu_solve_matrix, source/matrix_gpu.cu:263-307).  Solves A x = b for one row-major n x n FP64
+ * matrix with GSL's partial-pivot LU operation order on the GPU.  Returns
+ *   0  ok
+ *   2  some row with b_i > 0 fails the relative residual check |Ax-b|_i/|Ax|_i > 1e-6 (x valid)
+ *   3  some row with b_i <= 0 fails the absolute residual check |Ax-b|_i > 1e-6 (x valid)
+ *   4  singular (an exact zero pivot); x is not written
+ *   5  the GPU library failed (bad argument, no device, out of memory); x is not written
+ * a_matrix and b_matrix are only read (the reference's CPU path overwrites a_matrix with the
+ * LU factors; callers free it straight after, source/matrix_ion.c:284-285). */
+int solve_matrix (double *a_matrix, double *b_matrix, int size, double *x_matrix, int nplasma);
+
+/* Replaces invert_matrix, source/matrix_cpu.c:337-349 (-> cpu_invert_matrix :223-257 or
+ * gpu_invert_matrix, source/matrix_gpu.cu:331-384).  Writes the row-major inverse.  Returns
+ * 0, 1 (GSL_EDOM: singular, as gsl_linalg_LU_invx, GSL 2.6 linalg/lu.c:447-449) or 5 (GPU
+ * library failure).  The output is untouched on error, as in the reference. */
+int invert_matrix (double *matrix, double *inverted_matrix, int num_rows);
+
+/* Replaces get_matrix_error_string, source/matrix_cpu.c:273-281 (strings :58-72). */
+const char *get_matrix_error_string (int error_code);
+
+/* Replace cusolver_create / cusolver_destroy (source/matrix_gpu.h:16-17, matrix_gpu.cu:124-169,
+ * called at source/python.c:101-108,749-751).  hipla_init is optional: every entry point
+ * initialises on first use.  Return 0 on success. */
+int hipla_init (void);
+int hipla_finalize (void);
+
+/* Text of the last library-level failure in this process (code 5 above), or "". */
+const char *hipla_last_error_detail (void);
+
+/* ------------------------------------------------------------------------------------------ *
+ * Batched ragged API (new; the reference solves one matrix per call).  It batches the per-cell *
+ * loops that call the path: wind_update's cell loop (source/wind_updates2d.c:89-153 ->        *
+ * matrix_ion.c:276) and calc_all_matom_matrices (source/macro_accelerate.c:1052-1061 -> :315). *
+ *                                                                                             *
+ * Cell c is the row-major n[c] x n[c] FP64 matrix at A + offA[c] (in doubles) with its        *
+ * right-hand side at B + offV[c] and solution at X + offV[c].  A plan is built once per batch  *
+ * shape from HOST arrays n / offA / offV, then executed any number of times on DEVICE data.   *
+ * ------------------------------------------------------------------------------------------ */
+
+enum
+{
+  HIPLA_MODE_FP64 = 0,  /* GSL operation order in FP64: equal to the reference's results        */
+  HIPLA_MODE_MIXED = 1  /* equilibrated FP32 LU + FP64 residual refinement + FP64 LU fallback     */
+};
+
+enum
+{
+  HIPLA_FLAG_GRAPH = 1,         /* capture the launch sequence in a hipGraph and replay it       */
+  HIPLA_FLAG_KEEP_FACTORS = 2,  /* keep LU factors and pivots in the plan (for hipla_adjoint)    */
+  HIPLA_FLAG_NO_FALLBACK = 4,   /* mixed mode: report stalled cells instead of re-solving them   */
+  HIPLA_FLAG_PER_CELL = 8       /* plain per-matrix launches: one kernel launch per cell         */
+};
+
+/* Per-cell report of the mixed solver (device array of ncells records). */
+typedef struct
+{
+  int32_t refine_steps; /* FP64 residual refinement steps taken (0..max_refine)                     */
+  int32_t fallback;     /* 1 if the cell was re-solved by the FP64 LU after refinement stalled      */
+  double rel_correction;/* ||dx||_inf / ||x||_inf of the last refinement step                      */
+  double backward_err;  /* ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) at exit, in FP64     */
+} hipla_cell_stats;
+
+typedef struct hipla_plan_s *hipla_plan;
+
+/* Build a plan for ncells cells; n, offA, offV are host arrays.  Returns 0 or 5. */
+int hipla_plan_create (hipla_plan *plan, int ncells, const int *n, const int64_t *offA, const int64_t *offV,
+                       int mode, int flags);
+int hipla_plan_destroy (hipla_plan plan);
+
+/* Mixed-mode knobs: refinement step cap (default 5) and the ||dx||/||x|| target (default 1e-13). */
+int hipla_plan_set_refinement (hipla_plan plan, int max_refine, double tol);
+
+/* Solve every cell on `stream` (a hipStream_t, or NULL for the default stream).  dA, dB, dX,
+ * dcodes (ncells ints, solve_matrix's codes per cell) and dstats (may be NULL) are device
+ * pointers.  Asynchronous: returns 0 once queued, or 5. */
+int hipla_solve (hipla_plan plan, const double *dA, const double *dB, double *dX, int *dcodes,
+                 hipla_cell_stats *dstats, void *stream);
+
+/* Invert every cell (offV is ignored); dInv has A's layout.  dcodes gets invert_matrix's codes
+ * per cell.  HIPLA_MODE_MIXED uses BF16x3 MFMA products for the blocked LU and inverse and
+ * refines the inverse with FP64 residuals. */
+int hipla_invert (hipla_plan plan, const double *dA, double *dInv, int *dcodes, hipla_cell_stats *dstats,
+                  void *stream);
+
+/* Adjoint of the last hipla_solve on this plan (needs HIPLA_FLAG_KEEP_FACTORS, HIPLA_MODE_FP64):
+ * solves A^T lambda = g per cell with the stored LU factors and writes dL/dA = -lambda x^T
+ * (layout of A) into dGradA when it is not NULL.  dG, dLambda have B's layout. */
+int hipla_adjoint (hipla_plan plan, const double *dX, const double *dG, double *dLambda, double *dGradA,
+                   void *stream);
+
+/* Time the kernel variants for this plan's size histogram and keep the fastest per size class
+ * (the choice is cached per process and, if HIPLA_TUNE_CACHE names a file, across processes).
+ * Runs synchronously on `stream`.  dA..dX must be valid inputs as for hipla_solve. */
+int hipla_plan_autotune (hipla_plan plan, const double *dA, const double *dB, double *dX, int *dcodes,
+                         void *stream);
+
+/* Describe the plan's schedule (size classes, kernel variant per class, tiles) into buf. */
+int hipla_plan_describe (hipla_plan plan, char *buf, int buflen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HIPLA_H */

@@ -1,0 +1,158 @@
+"""CPU oracle for the per-cell dense solve / inverse path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` import
+this package, and only as the checker.  The product (``python_amd``, ``libhipla.so``) never
+imports, links or calls it.
+
+The C restatement lives in ``hipla_ref.c`` (each function cites the reference file:line it
+follows: ``source/matrix_cpu.c`` and GSL 2.6 ``linalg/lu.c``, ``invtri.c``, ``trimult.c`` and
+the cblas kernels).  This module loads ``liboracle.so`` built from it by ``oracle/Makefile``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_szp = ctypes.POINTER(ctypes.c_size_t)
+_up = ctypes.POINTER(ctypes.c_uint)
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with gcc (oracle/Makefile)."""
+    if force or not os.path.exists(_LIB):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
+    return _LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        L.ref_lu_decomp.argtypes = [_dp, ctypes.c_int, _szp, _up, _ip]
+        L.ref_lu_decomp_rightlooking.argtypes = [_dp, ctypes.c_int, _szp, _up, _ip]
+        L.ref_skip_window.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip]
+        L.ref_skip_window.restype = None
+        L.ref_lu_solve.argtypes = [_dp, ctypes.c_int, _szp, _dp, _dp]
+        L.ref_residual_code.argtypes = [_dp, ctypes.c_int, _dp, _dp]
+        L.ref_solve_matrix.argtypes = [_dp, _dp, ctypes.c_int, _dp]
+        L.ref_lu_invx.argtypes = [_dp, ctypes.c_int, _szp]
+        L.ref_invert_matrix.argtypes = [_dp, _dp, ctypes.c_int]
+        L.ref_lu_solve_transpose.argtypes = [_dp, ctypes.c_int, _szp, _dp, _dp]
+        L.ref_solve_dd_refined.argtypes = [_dp, _dp, ctypes.c_int, _dp, ctypes.c_int]
+        L.ref_solve_batch.argtypes = [ctypes.c_int, _ip, _i64p, _i64p, _dp, _dp, _dp, _ip, ctypes.c_int]
+        L.ref_invert_batch.argtypes = [ctypes.c_int, _ip, _i64p, _dp, _dp, _ip, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _c(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def lu_decomp(a, rightlooking: bool = False):
+    """GSL LU_decomp: returns (LU, perm, ipiv, signum); perm as gsl_permutation.data."""
+    a = _c(a).copy()
+    n = a.shape[0]
+    perm = np.zeros(n, dtype=np.uintp)
+    ipiv = np.zeros(n, dtype=np.uint32)
+    s = ctypes.c_int(0)
+    f = lib().ref_lu_decomp_rightlooking if rightlooking else lib().ref_lu_decomp
+    f(_d(a), n, perm.ctypes.data_as(_szp), ipiv.ctypes.data_as(_up), ctypes.byref(s))
+    return a, perm.astype(np.int64), ipiv.astype(np.int64), s.value
+
+
+def skip_window(n: int, k: int, pos: int):
+    lo, hi = ctypes.c_int(), ctypes.c_int()
+    lib().ref_skip_window(n, k, pos, ctypes.byref(lo), ctypes.byref(hi))
+    return lo.value, hi.value
+
+
+def lu_solve(lu, perm, b):
+    lu = _c(lu)
+    n = lu.shape[0]
+    p = np.ascontiguousarray(perm, dtype=np.uintp)
+    x = np.zeros(n)
+    st = lib().ref_lu_solve(_d(lu), n, p.ctypes.data_as(_szp), _d(_c(b)), _d(x))
+    return st, x
+
+
+def solve_matrix(a, b):
+    """cpu_solve_matrix semantics: returns (code, x); x is zeros when code == 4."""
+    a = _c(a).copy()
+    b = _c(b)
+    n = b.shape[0]
+    x = np.zeros(n)
+    code = lib().ref_solve_matrix(_d(a), _d(b), n, _d(x))
+    return code, x
+
+
+def residual_code(a, b, x):
+    a, b, x = _c(a), _c(b), _c(x)
+    return lib().ref_residual_code(_d(a), b.shape[0], _d(b), _d(x))
+
+
+def invert_matrix(m):
+    """cpu_invert_matrix semantics: returns (code, inverse or None)."""
+    m = _c(m).copy()
+    n = m.shape[0]
+    inv = np.zeros((n, n))
+    code = lib().ref_invert_matrix(_d(m), _d(inv), n)
+    return code, (inv if code == 0 else None)
+
+
+def lu_solve_transpose(lu, perm, g):
+    lu = _c(lu)
+    n = lu.shape[0]
+    p = np.ascontiguousarray(perm, dtype=np.uintp)
+    lam = np.zeros(n)
+    st = lib().ref_lu_solve_transpose(_d(lu), n, p.ctypes.data_as(_szp), _d(_c(g)), _d(lam))
+    return st, lam
+
+
+def solve_dd_refined(a, b, iters: int = 4):
+    a, b = _c(a), _c(b)
+    n = b.shape[0]
+    x = np.zeros(n)
+    st = lib().ref_solve_dd_refined(_d(a), _d(b), n, _d(x), iters)
+    return st, x
+
+
+def solve_batch(nsz, offA, offV, A, B, nthreads: int = 1):
+    """Ragged batch of cpu_solve_matrix calls; returns (X, codes)."""
+    nsz = np.ascontiguousarray(nsz, dtype=np.int32)
+    offA = np.ascontiguousarray(offA, dtype=np.int64)
+    offV = np.ascontiguousarray(offV, dtype=np.int64)
+    A, B = _c(A), _c(B)
+    X = np.zeros_like(B)
+    codes = np.zeros(len(nsz), dtype=np.int32)
+    lib().ref_solve_batch(len(nsz), nsz.ctypes.data_as(_ip), offA.ctypes.data_as(_i64p),
+                          offV.ctypes.data_as(_i64p), _d(A), _d(B), _d(X), codes.ctypes.data_as(_ip),
+                          int(nthreads))
+    return X, codes
+
+
+def invert_batch(nsz, offA, A, nthreads: int = 1):
+    nsz = np.ascontiguousarray(nsz, dtype=np.int32)
+    offA = np.ascontiguousarray(offA, dtype=np.int64)
+    A = _c(A)
+    Inv = np.zeros_like(A)
+    codes = np.zeros(len(nsz), dtype=np.int32)
+    lib().ref_invert_batch(len(nsz), nsz.ctypes.data_as(_ip), offA.ctypes.data_as(_i64p), _d(A), _d(Inv),
+                           codes.ctypes.data_as(_ip), int(nthreads))
+    return Inv, codes

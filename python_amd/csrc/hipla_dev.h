@@ -1,0 +1,218 @@
+// hipla_dev.h -- device helpers shared by the gfx950 kernels of libhipla.
+//
+// Wave64 reductions, the pivot key of GSL's idamax, the zero-multiplier skip window of GSL's
+// recursive LU, and the argument blocks the host side passes to the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+#define HIPLA_WAVE 64
+
+// ----------------------------------------------------------------------------------------
+// Kernel argument blocks
+// ----------------------------------------------------------------------------------------
+struct SolveArgs
+{
+  const int *n;          // per-cell order
+  const int64_t *offA;   // per-cell offset of A (doubles)
+  const int64_t *offV;   // per-cell offset of b / x (doubles)
+  const double *A;
+  const double *B;
+  double *X;
+  int *codes;
+  const int *list;       // cells handled by this launch, largest first
+  int nlist;
+  // optional LU factor store (HIPLA_FLAG_KEEP_FACTORS): LU row-major at LU + offA[c],
+  // GSL ipiv (row swapped with step k) at piv + offV[c]
+  double *LU;
+  int *piv;
+};
+
+// ----------------------------------------------------------------------------------------
+// Pivot key.  GSL's idamax (cblas/source_iamax_r.h:30-35) starts from max = 0 at the first
+// index and replaces it only on a strictly larger |x|; NaN never wins.  Mapping |x| to its
+// IEEE bit pattern (monotone for non-negative doubles) with NaN and 0 both mapped to 0, the
+// GSL choice is: the largest key, ties to the smallest current row position.
+// ----------------------------------------------------------------------------------------
+__device__ __forceinline__ long long pivot_key (double v)
+{
+  const double a = fabs (v);
+  return (a > 0.0) ? __double_as_longlong (a) : 0LL;
+}
+
+__device__ __forceinline__ int pivot_key32 (float v)
+{
+  const float a = fabsf (v);
+  return (a > 0.0f) ? __float_as_int (a) : 0;
+}
+
+// ----------------------------------------------------------------------------------------
+// Wave64 butterfly reductions (every lane gets the result).
+// ----------------------------------------------------------------------------------------
+__device__ __forceinline__ long long wave_max_i64 (long long v)
+{
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1)
+  {
+    long long o = __shfl_xor (v, m);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ int wave_min_i32 (int v)
+{
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1)
+  {
+    int o = __shfl_xor (v, m);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ int wave_max_i32 (int v)
+{
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1)
+  {
+    int o = __shfl_xor (v, m);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ double wave_max_f64 (double v)
+{
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1)
+  {
+    double o = __shfl_xor (v, m);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum_f64 (double v)
+{
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1)
+    v += __shfl_xor (v, m);
+  return v;
+}
+
+__device__ __forceinline__ float wave_max_f32 (float v)
+{
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1)
+  {
+    float o = __shfl_xor (v, m);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// (key, pos) lexicographic: larger key first, then smaller position.
+__device__ __forceinline__ void wave_argmax_keypos (long long &key, int &pos)
+{
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1)
+  {
+    long long ok = __shfl_xor (key, m);
+    int op = __shfl_xor (pos, m);
+    if (ok > key || (ok == key && op < pos))
+    {
+      key = ok;
+      pos = op;
+    }
+  }
+}
+
+__device__ __forceinline__ void wave_argmax_keypos32 (int &key, int &pos)
+{
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1)
+  {
+    int ok = __shfl_xor (key, m);
+    int op = __shfl_xor (pos, m);
+    if (ok > key || (ok == key && op < pos))
+    {
+      key = ok;
+      pos = op;
+    }
+  }
+}
+
+// Orders this wave's LDS accesses: every LDS op issued before completes before any after.
+__device__ __forceinline__ void wave_lds_fence ()
+{
+  __asm__ volatile ("s_waitcnt lgkmcnt(0)" ::: "memory");
+}
+
+// ----------------------------------------------------------------------------------------
+// Zero-multiplier skip window of GSL's recursive LU (linalg/lu.c:204-288, recurse.h:2,6-7).
+//
+// GSL reaches a(pos, j) from step k through dger inside a leaf panel, dtrsm above a split
+// boundary, or dgemm below it; only dgemm (cblas/source_gemm_r.h:76) skips a zero multiplier.
+// For step k the dgemm columns of row position pos are [lo, hi): lo = end of k's leaf panel,
+// hi = the first split boundary on k's path above pos (n if none).  Skipping only preserves
+// signs of zeros and keeps inf*0 out, so the rule matters for bit patterns, not values.
+// ----------------------------------------------------------------------------------------
+__device__ __forceinline__ int gsl_split (int n)
+{
+  return (n >= 16) ? ((n + 8) / 16) * 8 : n / 2;
+}
+
+// Walk k's path down the recursion.  Boundaries where k goes left are met in decreasing order,
+// so the last one above pos is the smallest one above pos.
+__device__ __forceinline__ void skip_window (int n, int k, int pos, int &lo, int &hi)
+{
+  int c = 0, N = n, h = n;
+  while (N > 24)
+  {
+    const int N1 = gsl_split (N);
+    const int b = c + N1;
+    if (k < b)
+    {
+      if (b > pos)
+        h = b;
+      N = N1;
+    }
+    else
+    {
+      c = b;
+      N -= N1;
+    }
+  }
+  lo = c + N;
+  hi = h;
+}
+
+// GSL's multiplier: reciprocal multiply when |pivot| >= DBL_MIN, else divide (linalg/lu.c:162-176).
+__device__ __forceinline__ double gsl_multiplier (double a, double piv)
+{
+  if (fabs (piv) >= DBL_MIN)
+  {
+    const double s = 1.0 / piv;
+    return a * s;
+  }
+  return a / piv;
+}
+
+// One step of the residual test of source/matrix_cpu.c:162-185 for row i: 0 (pass), 2, 3.
+__device__ __forceinline__ int residual_row_code (double test_val, double b)
+{
+  if (b > 0.0)
+    return (fabs ((test_val - b) / test_val) > 1.e-6) ? 2 : 0;
+  return (fabs (test_val - b) > 1.e-6) ? 3 : 0;
+}
+
+// Uniform (wave-wide) read of a 64-bit value from lane `src`.
+__device__ __forceinline__ double wave_bcast_f64 (double v, int src)
+{
+  int lo = __builtin_amdgcn_readlane (__double2loint (v), src);
+  int hi = __builtin_amdgcn_readlane (__double2hiint (v), src);
+  return __hiloint2double (hi, lo);
+}

@@ -1,0 +1,104 @@
+// hipla_plan.h -- host-side structures of libhipla: device info, error macro, batch plans.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/hipla.h"
+
+struct DeviceInfo
+{
+  int device = 0;
+  int num_cu = 256;
+  size_t max_smem = 65536;
+};
+
+const DeviceInfo &hipla_device_info ();
+int hipla_ensure_init ();
+void hipla_set_detail (const char *fmt, ...);
+
+#define HIPLA_CHECK(expr)                                                                        \
+  do                                                                                             \
+  {                                                                                              \
+    hipError_t _e = (expr);                                                                      \
+    if (_e != hipSuccess)                                                                        \
+    {                                                                                            \
+      hipla_set_detail ("%s failed: %s (%s:%d)", #expr, hipGetErrorString (_e), __FILE__, __LINE__); \
+      return 5;                                                                                  \
+    }                                                                                            \
+  } while (0)
+
+// Kernel shapes a size class can be routed to.
+enum KernelKind
+{
+  KK_WAVE_F64 = 0,     // k_wave_f64<NC>            one wave per matrix, FP64 GSL order
+  KK_LDS_F64 = 1,      // k_lds_f64                 workgroup per matrix, [A|b] in LDS
+  KK_BLOCKED_F64 = 2,  // k_blocked_f64             blocked, panel in LDS, HBM workspace
+  KK_WAVE_MIX = 3,     // k_wave_mixed<NC>          FP32 LU in registers + FP64 refinement
+  KK_LDS_MIX = 4,      // k_lds_mixed               FP32 LU in LDS + FP64 refinement
+  KK_BLOCKED_MIX = 5,  // k_blocked_mixed           FP32 blocked LU in HBM workspace + refinement
+  KK_COUNT = 6
+};
+
+// A group of cells that one launch handles (cells sorted largest first).
+struct Group
+{
+  int kind = 0;
+  int param = 0;      // NC for wave kernels, panel width for blocked ones
+  int nmax = 0;       // largest order in the group
+  int first = 0;      // offset into the plan's cell list
+  int count = 0;
+  int grid = 0;       // persistent grid size (blocked kernels)
+  int stream = 0;     // fork-join branch
+};
+
+struct hipla_plan_s
+{
+  int ncells = 0;
+  int mode = 0;
+  int flags = 0;
+  int max_refine = 5;
+  double tol = 1e-13;
+  std::vector<int> n;
+  std::vector<int64_t> offA, offV;
+  int64_t totA = 0, totV = 0;
+  std::vector<int> list;     // cells, grouped
+  std::vector<Group> groups;
+  // device copies
+  int *d_n = nullptr;
+  int64_t *d_offA = nullptr, *d_offV = nullptr;
+  int *d_list = nullptr;
+  // workspaces
+  double *d_work = nullptr;   // blocked kernels: per-workgroup slot
+  int64_t wstride = 0;
+  int work_slots = 0;
+  float *d_work32 = nullptr;  // mixed blocked kernels
+  int64_t wstride32 = 0;
+  int *d_fallback = nullptr;  // mixed: [0] = count, [1..] = cells to re-solve in FP64
+  double *d_LU = nullptr;     // keep-factors store (A's layout)
+  int *d_piv = nullptr;       // keep-factors pivots (B's layout)
+  // fork-join streams / events
+  std::vector<hipStream_t> aux;
+  std::vector<hipEvent_t> ev;
+  // graph cache
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  const void *gkey[6] = { nullptr, nullptr, nullptr, nullptr, nullptr, nullptr };
+  int gkind = -1;
+  // last solve (adjoint)
+  const double *last_A = nullptr;
+  int solved = 0;
+};
+
+// launch every group of a solve / invert on `st` (fork-join over aux streams)
+int plan_enqueue_solve (hipla_plan p, const double *dA, const double *dB, double *dX, int *dcodes,
+                        hipla_cell_stats *dstats, hipStream_t st);
+int plan_enqueue_invert (hipla_plan p, const double *dA, double *dInv, int *dcodes, hipla_cell_stats *dstats,
+                         hipStream_t st);
+
+// kernel-family attribute setup (dynamic LDS above 64 KB)
+hipError_t setup_mixed_attrs (size_t max_smem);
+hipError_t setup_invert_attrs (size_t max_smem);

@@ -1,0 +1,446 @@
+// hipla_plan.hip -- batched ragged plans: size-class routing, fork-join launch, hipGraph replay.
+//
+// The reference solves one cell per call inside its per-cell loops (source/wind_updates2d.c:89-153,
+// source/macro_accelerate.c:1052-1061).  A plan takes the whole ragged batch at once: every cell is
+// routed by its order n to a kernel shape (wave-per-matrix, LDS-resident, or blocked), each shape's
+// cells are sorted largest first so the hardware dispatcher approximates longest-processing-time
+// scheduling, the shapes run as concurrent branches of one fork-join launch, and the launch
+// sequence is captured once in a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "hipla_kernels.h"
+#include "hipla_plan.h"
+#include "hipla_tune.h"
+
+static const char *kind_name (int k)
+{
+  switch (k)
+  {
+  case KK_WAVE_F64:
+    return "wave_f64";
+  case KK_LDS_F64:
+    return "lds_f64";
+  case KK_BLOCKED_F64:
+    return "blocked_f64";
+  case KK_WAVE_MIX:
+    return "wave_mixed";
+  case KK_LDS_MIX:
+    return "lds_mixed";
+  case KK_BLOCKED_MIX:
+    return "blocked_mixed";
+  default:
+    return "?";
+  }
+}
+
+// Largest order the LDS-resident FP64 kernel can hold ([A|b] plus two work vectors).
+static int lds_f64_limit ()
+{
+  const size_t cap = hipla_device_info ().max_smem - 2048;
+  int n = 1;
+  while (lds_f64_smem (n + 1) <= cap)
+    n++;
+  return n;
+}
+
+static void route_fp64 (int n, int &kind, int &param)
+{
+  if (n <= 16)
+  {
+    kind = KK_WAVE_F64;
+    param = 16;
+  }
+  else if (n <= 32)
+  {
+    kind = KK_WAVE_F64;
+    param = 32;
+  }
+  else if (n <= 64)
+  {
+    kind = KK_WAVE_F64;
+    param = 64;
+  }
+  else if (n <= lds_f64_limit ())
+  {
+    kind = KK_LDS_F64;
+    param = 0;
+  }
+  else
+  {
+    kind = KK_BLOCKED_F64;
+    param = BLK_NB;
+  }
+}
+
+static int free_plan_device (hipla_plan p)
+{
+  if (p->gexec)
+    (void) hipGraphExecDestroy (p->gexec);
+  if (p->graph)
+    (void) hipGraphDestroy (p->graph);
+  p->gexec = nullptr;
+  p->graph = nullptr;
+  void *bufs[] = { p->d_n, p->d_offA, p->d_offV, p->d_list, p->d_work, p->d_work32, p->d_fallback, p->d_LU, p->d_piv };
+  for (void *b : bufs)
+    if (b)
+      (void) hipFree (b);
+  for (auto s : p->aux)
+    (void) hipStreamDestroy (s);
+  for (auto e : p->ev)
+    (void) hipEventDestroy (e);
+  p->aux.clear ();
+  p->ev.clear ();
+  return 0;
+}
+
+extern "C" int hipla_plan_destroy (hipla_plan plan)
+{
+  if (!plan)
+    return 0;
+  free_plan_device (plan);
+  delete plan;
+  return 0;
+}
+
+extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, const int64_t *offA, const int64_t *offV,
+                                  int mode, int flags)
+{
+  if (!out || ncells <= 0 || !n || !offA || (!offV && ncells > 0))
+  {
+    hipla_set_detail ("hipla_plan_create: bad argument");
+    return 5;
+  }
+  if (mode != HIPLA_MODE_FP64 && mode != HIPLA_MODE_MIXED)
+  {
+    hipla_set_detail ("hipla_plan_create: unknown mode %d", mode);
+    return 5;
+  }
+  int rc = hipla_ensure_init ();
+  if (rc)
+    return rc;
+  *out = nullptr;
+  hipla_plan p = new hipla_plan_s ();
+  p->ncells = ncells;
+  p->mode = mode;
+  p->flags = flags;
+  p->n.assign (n, n + ncells);
+  p->offA.assign (offA, offA + ncells);
+  p->offV.assign (offV, offV + ncells);
+  for (int c = 0; c < ncells; c++)
+  {
+    if (n[c] <= 0 || n[c] > 65536 || offA[c] < 0 || offV[c] < 0)
+    {
+      hipla_set_detail ("hipla_plan_create: cell %d has n=%d offA=%lld offV=%lld", c, n[c], (long long) offA[c],
+                        (long long) offV[c]);
+      delete p;
+      return 5;
+    }
+    p->totA = std::max<int64_t> (p->totA, offA[c] + (int64_t) n[c] * n[c]);
+    p->totV = std::max<int64_t> (p->totV, offV[c] + n[c]);
+  }
+
+  // ---- route every cell to a (kind, param) group ----
+  std::vector<int> kind (ncells), param (ncells);
+  for (int c = 0; c < ncells; c++)
+  {
+    if (mode == HIPLA_MODE_MIXED)
+      route_mixed (n[c], kind[c], param[c]);
+    else
+      route_fp64 (n[c], kind[c], param[c]);
+  }
+  std::vector<int> idx (ncells);
+  std::iota (idx.begin (), idx.end (), 0);
+  std::stable_sort (idx.begin (), idx.end (), [&] (int a, int b) {
+    if (kind[a] != kind[b])
+      return kind[a] < kind[b];
+    if (param[a] != param[b])
+      return param[a] < param[b];
+    return n[a] > n[b];
+  });
+  p->list = idx;
+  const DeviceInfo &dev = hipla_device_info ();
+  int64_t wneed = 0, wneed32 = 0;
+  int wslots = 0, wslots32 = 0;
+  for (int i = 0; i < ncells;)
+  {
+    int j = i;
+    while (j < ncells && kind[idx[j]] == kind[idx[i]] && param[idx[j]] == param[idx[i]])
+      j++;
+    Group g;
+    g.kind = kind[idx[i]];
+    g.param = param[idx[i]];
+    g.first = i;
+    g.count = j - i;
+    g.nmax = n[idx[i]];
+    g.stream = (int) p->groups.size ();
+    if (g.kind == KK_BLOCKED_F64)
+    {
+      g.grid = std::min (g.count, 2 * dev.num_cu);
+      wneed = std::max<int64_t> (wneed, (int64_t) g.nmax * g.nmax + g.nmax);
+      wslots = std::max (wslots, g.grid);
+    }
+    if (g.kind == KK_BLOCKED_MIX)
+    {
+      g.grid = std::min (g.count, 2 * dev.num_cu);
+      wneed32 = std::max<int64_t> (wneed32, blocked_mixed_ws_floats (g.nmax));
+      wslots32 = std::max (wslots32, g.grid);
+    }
+    p->groups.push_back (g);
+    i = j;
+  }
+
+  // ---- device copies and workspaces ----
+#define PCHECK(expr)                                                                       \
+  do                                                                                       \
+  {                                                                                        \
+    hipError_t _e = (expr);                                                                \
+    if (_e != hipSuccess)                                                                  \
+    {                                                                                      \
+      hipla_set_detail ("%s failed: %s", #expr, hipGetErrorString (_e));                   \
+      free_plan_device (p);                                                                \
+      delete p;                                                                            \
+      return 5;                                                                            \
+    }                                                                                      \
+  } while (0)
+  PCHECK (hipMalloc (&p->d_n, sizeof (int) * ncells));
+  PCHECK (hipMalloc (&p->d_offA, sizeof (int64_t) * ncells));
+  PCHECK (hipMalloc (&p->d_offV, sizeof (int64_t) * ncells));
+  PCHECK (hipMalloc (&p->d_list, sizeof (int) * ncells));
+  PCHECK (hipMemcpy (p->d_n, n, sizeof (int) * ncells, hipMemcpyHostToDevice));
+  PCHECK (hipMemcpy (p->d_offA, offA, sizeof (int64_t) * ncells, hipMemcpyHostToDevice));
+  PCHECK (hipMemcpy (p->d_offV, offV, sizeof (int64_t) * ncells, hipMemcpyHostToDevice));
+  PCHECK (hipMemcpy (p->d_list, p->list.data (), sizeof (int) * ncells, hipMemcpyHostToDevice));
+  if (wslots > 0)
+  {
+    p->wstride = (wneed + 63) & ~(int64_t) 63;
+    p->work_slots = wslots;
+    PCHECK (hipMalloc (&p->d_work, sizeof (double) * p->wstride * wslots));
+  }
+  if (wslots32 > 0)
+  {
+    p->wstride32 = (wneed32 + 63) & ~(int64_t) 63;
+    PCHECK (hipMalloc (&p->d_work32, sizeof (float) * p->wstride32 * wslots32));
+  }
+  if (mode == HIPLA_MODE_MIXED)
+    PCHECK (hipMalloc (&p->d_fallback, sizeof (int) * (ncells + 1)));
+  if (flags & HIPLA_FLAG_KEEP_FACTORS)
+  {
+    PCHECK (hipMalloc (&p->d_LU, sizeof (double) * p->totA));
+    PCHECK (hipMalloc (&p->d_piv, sizeof (int) * p->totV));
+  }
+  const int nbranch = (int) p->groups.size ();
+  p->aux.resize (nbranch + 1, nullptr);
+  p->ev.resize (nbranch + 1, nullptr);
+  for (int b = 0; b <= nbranch; b++)
+  {
+    PCHECK (hipStreamCreateWithFlags (&p->aux[b], hipStreamNonBlocking));
+    PCHECK (hipEventCreateWithFlags (&p->ev[b], hipEventDisableTiming));
+  }
+#undef PCHECK
+  *out = p;
+  return 0;
+}
+
+extern "C" int hipla_plan_set_refinement (hipla_plan plan, int max_refine, double tol)
+{
+  if (!plan || max_refine < 0 || max_refine > 50 || !(tol > 0.0))
+  {
+    hipla_set_detail ("hipla_plan_set_refinement: bad argument");
+    return 5;
+  }
+  plan->max_refine = max_refine;
+  plan->tol = tol;
+  return 0;
+}
+
+extern "C" int hipla_plan_describe (hipla_plan p, char *buf, int buflen)
+{
+  if (!p || !buf || buflen <= 0)
+    return 5;
+  std::string s;
+  char line[256];
+  snprintf (line, sizeof line, "{\"ncells\": %d, \"mode\": %d, \"flags\": %d, \"groups\": [", p->ncells, p->mode,
+            p->flags);
+  s += line;
+  for (size_t i = 0; i < p->groups.size (); i++)
+  {
+    const Group &g = p->groups[i];
+    snprintf (line, sizeof line, "%s{\"kernel\": \"%s\", \"param\": %d, \"nmax\": %d, \"count\": %d, \"grid\": %d}",
+              i ? ", " : "", kind_name (g.kind), g.param, g.nmax, g.count, g.grid);
+    s += line;
+  }
+  s += "]}";
+  snprintf (buf, (size_t) buflen, "%s", s.c_str ());
+  return 0;
+}
+
+// --------------------------------------------------------------------------------------------
+// enqueue
+// --------------------------------------------------------------------------------------------
+static SolveArgs make_args (hipla_plan p, const double *dA, const double *dB, double *dX, int *dcodes,
+                            const Group &g)
+{
+  SolveArgs a;
+  a.n = p->d_n;
+  a.offA = p->d_offA;
+  a.offV = p->d_offV;
+  a.A = dA;
+  a.B = dB;
+  a.X = dX;
+  a.codes = dcodes;
+  a.list = p->d_list + g.first;
+  a.nlist = g.count;
+  a.LU = p->d_LU;
+  a.piv = p->d_piv;
+  return a;
+}
+
+static hipError_t launch_group_f64 (hipla_plan p, const SolveArgs &a, const Group &g, hipStream_t st)
+{
+  const bool keep = (p->flags & HIPLA_FLAG_KEEP_FACTORS) != 0;
+  switch (g.kind)
+  {
+  case KK_WAVE_F64:
+    return launch_wave_f64 (a, g.param, keep, st);
+  case KK_LDS_F64:
+    return launch_lds_f64 (a, g.nmax, keep, st);
+  case KK_BLOCKED_F64:
+    return launch_blocked_f64 (a, g.nmax, std::min (g.grid, std::max (a.nlist, 1)), p->d_work, p->wstride, keep, st);
+  default:
+    return hipErrorInvalidValue;
+  }
+}
+
+int plan_enqueue_solve (hipla_plan p, const double *dA, const double *dB, double *dX, int *dcodes,
+                        hipla_cell_stats *dstats, hipStream_t st)
+{
+  if (p->mode == HIPLA_MODE_MIXED)
+    return plan_enqueue_mixed (p, dA, dB, dX, dcodes, dstats, st);
+  const bool per_cell = (p->flags & HIPLA_FLAG_PER_CELL) != 0;
+  const int nb = (int) p->groups.size ();
+  // fork
+  if (nb > 1)
+    HIPLA_CHECK (hipEventRecord (p->ev[nb], st));
+  for (int gi = 0; gi < nb; gi++)
+  {
+    const Group &g = p->groups[gi];
+    hipStream_t s = st;
+    if (nb > 1 && gi > 0)
+    {
+      s = p->aux[gi];
+      HIPLA_CHECK (hipStreamWaitEvent (s, p->ev[nb], 0));
+    }
+    SolveArgs a = make_args (p, dA, dB, dX, dcodes, g);
+    if (per_cell)
+    {
+      // the reference's GPU path: one factor+solve launch per matrix (source/matrix_gpu.cu:263-307)
+      for (int c = 0; c < g.count; c++)
+      {
+        SolveArgs a1 = a;
+        a1.list = a.list + c;
+        a1.nlist = 1;
+        HIPLA_CHECK (launch_group_f64 (p, a1, g, s));
+      }
+    }
+    else
+      HIPLA_CHECK (launch_group_f64 (p, a, g, s));
+    if (nb > 1 && gi > 0)
+      HIPLA_CHECK (hipEventRecord (p->ev[gi], s));
+  }
+  // join
+  for (int gi = 1; gi < nb; gi++)
+    HIPLA_CHECK (hipStreamWaitEvent (st, p->ev[gi], 0));
+  return 0;
+}
+
+// --------------------------------------------------------------------------------------------
+// public execution entry points (graph capture / replay)
+// --------------------------------------------------------------------------------------------
+static int run_maybe_graph (hipla_plan p, int kind, const void *k0, const void *k1, const void *k2, const void *k3,
+                            const void *k4, hipStream_t st,
+                            int (*enqueue) (hipla_plan, const void *, const void *, const void *, const void *,
+                                            const void *, hipStream_t))
+{
+  if (!(p->flags & HIPLA_FLAG_GRAPH) || (p->flags & HIPLA_FLAG_PER_CELL))
+    return enqueue (p, k0, k1, k2, k3, k4, st);
+  const void *key[6] = { k0, k1, k2, k3, k4, (const void *) (intptr_t) kind };
+  bool hit = p->gexec != nullptr && p->gkind == kind;
+  for (int i = 0; hit && i < 6; i++)
+    hit = key[i] == p->gkey[i];
+  if (!hit)
+  {
+    if (p->gexec)
+      (void) hipGraphExecDestroy (p->gexec);
+    if (p->graph)
+      (void) hipGraphDestroy (p->graph);
+    p->gexec = nullptr;
+    p->graph = nullptr;
+    hipStream_t cap = p->aux.back ();
+    HIPLA_CHECK (hipStreamBeginCapture (cap, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue (p, k0, k1, k2, k3, k4, cap);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture (cap, &g);
+    if (rc)
+    {
+      if (g)
+        (void) hipGraphDestroy (g);
+      return rc;
+    }
+    HIPLA_CHECK (e);
+    p->graph = g;
+    HIPLA_CHECK (hipGraphInstantiate (&p->gexec, g, nullptr, nullptr, 0));
+    for (int i = 0; i < 6; i++)
+      p->gkey[i] = key[i];
+    p->gkind = kind;
+  }
+  HIPLA_CHECK (hipGraphLaunch (p->gexec, st));
+  return 0;
+}
+
+static int enq_solve (hipla_plan p, const void *a, const void *b, const void *x, const void *c, const void *s,
+                      hipStream_t st)
+{
+  return plan_enqueue_solve (p, (const double *) a, (const double *) b, (double *) x, (int *) c,
+                             (hipla_cell_stats *) s, st);
+}
+
+static int enq_invert (hipla_plan p, const void *a, const void *inv, const void *c, const void *s, const void *,
+                       hipStream_t st)
+{
+  return plan_enqueue_invert (p, (const double *) a, (double *) inv, (int *) c, (hipla_cell_stats *) s, st);
+}
+
+extern "C" int hipla_solve (hipla_plan p, const double *dA, const double *dB, double *dX, int *dcodes,
+                            hipla_cell_stats *dstats, void *stream)
+{
+  if (!p || !dA || !dB || !dX || !dcodes)
+  {
+    hipla_set_detail ("hipla_solve: bad argument");
+    return 5;
+  }
+  int rc = run_maybe_graph (p, 0, dA, dB, dX, dcodes, dstats, (hipStream_t) stream, enq_solve);
+  if (rc == 0)
+  {
+    p->last_A = dA;
+    p->solved = 1;
+  }
+  return rc;
+}
+
+extern "C" int hipla_invert (hipla_plan p, const double *dA, double *dInv, int *dcodes, hipla_cell_stats *dstats,
+                             void *stream)
+{
+  if (!p || !dA || !dInv || !dcodes)
+  {
+    hipla_set_detail ("hipla_invert: bad argument");
+    return 5;
+  }
+  return run_maybe_graph (p, 1, dA, dInv, dcodes, dstats, nullptr, (hipStream_t) stream, enq_invert);
+}

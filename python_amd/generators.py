@@ -1,0 +1,275 @@
+"""Seeded synthetic cell batches for the per-cell dense solve / inverse path.
+
+The reference builds its matrices from atomic physics (source/matrix_ion.c:444-659
+``populate_ion_rate_matrix``; source/macro_accelerate.c:30-348 ``calc_matom_matrix``).  These
+generators restate their *structure* with closed-form random rates:
+
+* Ionization balance (``matrix_ion.c``): per element a block of ionization stages; off-diagonal
+  A[i+1, i] = rate i -> i+1 (photo/collisional ionization), A[i, i+1] = n_e * alpha (recombination
+  i+1 -> i); every diagonal is minus its column sum (population conservation of the rate
+  equations); then the row of each element's first ion is replaced by the abundance constraint
+  (ones over that element's ions, b = abundance), as ``populate_ion_rate_matrix`` does
+  (matrix_ion.c:633-655).  b is zero except on those rows.
+* Macro-atom inverse (``macro_accelerate.c:269-284``): N = I - Q with Q substochastic.
+
+All randomness comes from ``numpy.random.default_rng(seed)`` so the same seed gives the same
+bytes on any machine.  Layout of a batch: cell c is the row-major n_c x n_c matrix at
+``A[offA[c]:offA[c]+n_c*n_c]`` and its right-hand side at ``B[offV[c]:offV[c]+n_c]``.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# (name, number of ionization stages incl. neutral, abundance relative to H)
+ELEMENTS = [
+    ("H", 2, 1.0), ("He", 3, 0.1), ("C", 7, 3e-4), ("N", 8, 1e-4), ("O", 9, 7e-4),
+    ("Ne", 11, 1e-4), ("Mg", 13, 4e-5), ("Si", 15, 3.5e-5), ("S", 17, 1.5e-5), ("Fe", 27, 3e-5),
+    ("Na", 12, 2e-6), ("Al", 14, 3e-6), ("Ar", 19, 4e-6), ("Ca", 21, 2e-6), ("Ni", 29, 2e-6),
+]
+SLACK_ABUNDANCE = 1e-5
+
+
+def element_layout(n: int):
+    """Split n unknowns into element blocks (start, stages, abundance) plus one slack row.
+
+    Elements are taken in ELEMENTS order (repeating the list if n needs more); the last block
+    is truncated so the blocks fill n - 1 rows exactly; the final row is a one-stage slack
+    element.  n = 30 gives H(2) He(3) C(7) N(8) O(9) + slack, the layout of the 900-cell case.
+    """
+    if n < 1:
+        raise ValueError("n must be >= 1")
+    blocks = []
+    s = 0
+    room = n - 1
+    i = 0
+    while room > 0:
+        name, m, ab = ELEMENTS[i % len(ELEMENTS)]
+        m = min(m, room)
+        blocks.append((s, m, ab))
+        s += m
+        room -= m
+        i += 1
+    blocks.append((s, 1, SLACK_ABUNDANCE))
+    return blocks
+
+
+def _logu(rng, lo, hi, size):
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), size))
+
+
+def ion_matrices(rng, ncells: int, n: int, ion_range=(1e-12, 1e2), ne_range=(1e6, 1e14),
+                 alpha_range=(1e-14, 1e-10)):
+    """ncells ionization-balance matrices of order n: returns (A [ncells,n,n], b [ncells,n])."""
+    A = np.zeros((ncells, n, n))
+    b = np.zeros((ncells, n))
+    ne = _logu(rng, *ne_range, ncells)
+    blocks = element_layout(n)
+    for (s, m, ab) in blocks:
+        if m > 1:
+            k = np.arange(m - 1)
+            ion = _logu(rng, *ion_range, (ncells, m - 1))
+            rec = ne[:, None] * _logu(rng, *alpha_range, (ncells, m - 1))
+            A[:, s + k + 1, s + k] = ion
+            A[:, s + k, s + k + 1] = rec
+    d = np.arange(n)
+    A[:, d, d] = -A.sum(axis=1)
+    for (s, m, ab) in blocks:
+        A[:, s, :] = 0.0
+        A[:, s, s:s + m] = 1.0
+        b[:, s] = ab
+    return A, b
+
+
+def ion_matrices_te(rng, ncells: int, n: int, te=None):
+    """Temperature-dependent ionization matrices of order n (the adjoint / Newton case).
+
+    Ionization i -> i+1: C * T^0.5 * exp(-E / kT), E ~ U[5, 500] eV, C log-uniform.
+    Recombination i+1 -> i: alpha0 * (T / 1e4)^-0.7, alpha0 log-uniform (n_e folded in).
+    Returns a dict with T, the rate parameters, A(T), dA/dT and b.
+    """
+    if te is None:
+        te = _logu(rng, 3e3, 1e7, ncells)
+    blocks = element_layout(n)
+    params = []
+    for (s, m, ab) in blocks:
+        if m > 1:
+            C = _logu(rng, 1e-8, 1e-2, (ncells, m - 1))
+            E = rng.uniform(5.0, 500.0, (ncells, m - 1))
+            a0 = _logu(rng, 1e-6, 1e2, (ncells, m - 1))
+            params.append((s, m, ab, C, E, a0))
+        else:
+            params.append((s, m, ab, None, None, None))
+    A, dA, b = te_assemble(n, blocks, params, te)
+    return {"te": te, "blocks": blocks, "params": params, "A": A, "dA": dA, "b": b}
+
+
+KB_EV = 8.617333262e-5  # Boltzmann constant, eV / K
+
+
+def te_assemble(n, blocks, params, te):
+    """A(T), dA/dT and b for ion_matrices_te parameters at temperatures te [ncells]."""
+    ncells = te.shape[0]
+    A = np.zeros((ncells, n, n))
+    dA = np.zeros((ncells, n, n))
+    b = np.zeros((ncells, n))
+    T = te[:, None]
+    for (s, m, ab, C, E, a0) in params:
+        if m > 1:
+            k = np.arange(m - 1)
+            x = E / (KB_EV * T)
+            ion = C * np.sqrt(T) * np.exp(-x)
+            dion = ion * (0.5 / T + x / T)
+            rec = a0 * (T / 1e4) ** -0.7
+            drec = rec * (-0.7 / T)
+            A[:, s + k + 1, s + k] = ion
+            A[:, s + k, s + k + 1] = rec
+            dA[:, s + k + 1, s + k] = dion
+            dA[:, s + k, s + k + 1] = drec
+    d = np.arange(n)
+    A[:, d, d] = -A.sum(axis=1)
+    dA[:, d, d] = -dA.sum(axis=1)
+    for (s, m, ab, *_rest) in params:
+        A[:, s, :] = 0.0
+        dA[:, s, :] = 0.0
+        A[:, s, s:s + m] = 1.0
+        b[:, s] = ab
+    return A, dA, b
+
+
+def macro_q(rng, ncells: int, n: int, density=(0.10, 0.40), val_range=(1e-8, 1.0), eps_range=(1e-6, 1e-1)):
+    """ncells macro-atom matrices N = I - Q of order n (Q substochastic, zero diagonal)."""
+    N = np.zeros((ncells, n, n))
+    for c in range(ncells):
+        dens = rng.uniform(*density, n)
+        mask = rng.random((n, n)) < dens[:, None]
+        np.fill_diagonal(mask, False)
+        Q = np.where(mask, _logu(rng, *val_range, (n, n)), 0.0)
+        rs = Q.sum(axis=1)
+        eps = _logu(rng, *eps_range, n)
+        scale = np.where(rs > 0, (1.0 - eps) / np.where(rs > 0, rs, 1.0), 0.0)
+        Q *= scale[:, None]
+        N[c] = -Q
+        N[c][np.arange(n), np.arange(n)] = 1.0
+    return N
+
+
+@dataclass
+class RaggedBatch:
+    """A ragged batch of square FP64 cells packed row-major back to back."""
+    n: np.ndarray                 # int32 [ncells]
+    offA: np.ndarray              # int64 [ncells]
+    offV: np.ndarray              # int64 [ncells]
+    A: np.ndarray                 # float64 [sum n^2]
+    B: np.ndarray                 # float64 [sum n]
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def ncells(self) -> int:
+        return int(self.n.shape[0])
+
+    def cell(self, c: int):
+        n = int(self.n[c])
+        a = self.A[self.offA[c]:self.offA[c] + n * n].reshape(n, n)
+        b = self.B[self.offV[c]:self.offV[c] + n]
+        return a, b
+
+    def subset(self, cells):
+        cells = list(cells)
+        mats = [self.cell(c) for c in cells]
+        return pack([m[0] for m in mats], [m[1] for m in mats])
+
+    @property
+    def bytes_solve(self) -> int:
+        """Algorithmic bytes of a solve: A and b in, x out (FP64)."""
+        n = self.n.astype(np.int64)
+        return int((8 * (n * n + 2 * n)).sum())
+
+    @property
+    def flops_solve(self) -> float:
+        n = self.n.astype(np.float64)
+        return float((2.0 / 3.0 * n ** 3 + 2.0 * n ** 2).sum())
+
+
+def pack(mats, rhs=None) -> RaggedBatch:
+    ns = np.array([m.shape[0] for m in mats], dtype=np.int32)
+    offA = np.zeros(len(mats), dtype=np.int64)
+    offV = np.zeros(len(mats), dtype=np.int64)
+    if len(mats):
+        offA[1:] = np.cumsum(ns.astype(np.int64) ** 2)[:-1]
+        offV[1:] = np.cumsum(ns.astype(np.int64))[:-1]
+    A = np.concatenate([np.ascontiguousarray(m, dtype=np.float64).ravel() for m in mats]) if mats else np.zeros(0)
+    if rhs is None:
+        B = np.zeros(int(ns.sum()))
+    else:
+        B = np.concatenate([np.ascontiguousarray(r, dtype=np.float64).ravel() for r in rhs])
+    return RaggedBatch(ns, offA, offV, A, B)
+
+
+def pack_groups(groups, order) -> RaggedBatch:
+    """Pack {n: (A[k,n,n], b[k,n])} groups into a batch whose cell c has order order[c]."""
+    ns = np.asarray(order, dtype=np.int32)
+    offA = np.zeros(len(ns), dtype=np.int64)
+    offV = np.zeros(len(ns), dtype=np.int64)
+    offA[1:] = np.cumsum(ns.astype(np.int64) ** 2)[:-1]
+    offV[1:] = np.cumsum(ns.astype(np.int64))[:-1]
+    A = np.empty(int((ns.astype(np.int64) ** 2).sum()))
+    B = np.empty(int(ns.astype(np.int64).sum()))
+    cursor = {k: 0 for k in groups}
+    for c, n in enumerate(ns):
+        Ag, bg = groups[int(n)]
+        i = cursor[int(n)]
+        cursor[int(n)] = i + 1
+        A[offA[c]:offA[c] + n * n] = Ag[i].ravel()
+        B[offV[c]:offV[c] + n] = bg[i]
+    return RaggedBatch(ns, offA, offV, A, B)
+
+
+# ------------------------------------------------------------------------------------------
+# The workloads
+# ------------------------------------------------------------------------------------------
+UNEVEN_SIZES = (12, 30, 64, 96, 128, 200, 256)
+UNEVEN_COUNTS = (3000, 2500, 1500, 1000, 1000, 500, 500)
+
+
+def plain_small(ncells: int = 900, seed: int = 1) -> RaggedBatch:
+    """900 cells of the n = 30 ionization matrix (H, He, C, N, O + slack), seed 1."""
+    rng = np.random.default_rng(seed)
+    A, b = ion_matrices(rng, ncells, 30)
+    batch = pack_groups({30: (A, b)}, [30] * ncells)
+    batch.meta = {"workload": "plain_small", "seed": seed}
+    return batch
+
+
+def uneven(ncells: int = 10000, seed: int = 2, wide: bool = False, scale: float = 1.0) -> RaggedBatch:
+    """10,000 cells, n in {12..256} with counts 3000/2500/1500/1000/1000/500/500, shuffled.
+
+    wide=True widens the rates to 1e-20..1e4 (the mixed-precision stress case, seed 3).
+    ``scale`` shrinks every count proportionally (for quick tests)."""
+    rng = np.random.default_rng(seed)
+    counts = [max(1, int(round(c * scale * ncells / 10000))) for c in UNEVEN_COUNTS]
+    order = np.concatenate([np.full(c, n, dtype=np.int32) for n, c in zip(UNEVEN_SIZES, counts)])
+    rng.shuffle(order)
+    groups = {}
+    for n, c in zip(UNEVEN_SIZES, counts):
+        if wide:
+            groups[n] = ion_matrices(rng, c, n, ion_range=(1e-20, 1e4), ne_range=(1e6, 1e14),
+                                     alpha_range=(1e-26, 1e-10))
+        else:
+            groups[n] = ion_matrices(rng, c, n)
+    batch = pack_groups(groups, order)
+    batch.meta = {"workload": "uneven_wide" if wide else "uneven", "seed": seed, "counts": dict(zip(UNEVEN_SIZES, counts))}
+    return batch
+
+
+def macro_inverse(ncells: int = 2000, seed: int = 4, sizes=(256, 512, 1024)) -> RaggedBatch:
+    """2,000 macro-atom cells N = I - Q, n in {256, 512, 1024} in equal thirds, seed 4."""
+    rng = np.random.default_rng(seed)
+    per = [ncells // len(sizes) + (1 if i < ncells % len(sizes) else 0) for i in range(len(sizes))]
+    order = np.concatenate([np.full(c, n, dtype=np.int32) for n, c in zip(sizes, per)])
+    rng.shuffle(order)
+    groups = {n: (macro_q(rng, c, n), np.zeros((c, n))) for n, c in zip(sizes, per)}
+    batch = pack_groups(groups, order)
+    batch.meta = {"workload": "macro_inverse", "seed": seed}
+    return batch
