@@ -88,7 +88,7 @@ int hipla_plan_create (hipla_plan *plan, int ncells, const int *n, const int64_t
                        int mode, int flags);
 int hipla_plan_destroy (hipla_plan plan);
 
-/* Mixed-mode knobs: refinement step cap (default 5) and the ||dx||/||x|| target (default 1e-13). */
+/* Mixed-mode knobs: refinement step cap (default 5) and the ||dx||/||x|| target (default 1e-12). */
 int hipla_plan_set_refinement (hipla_plan plan, int max_refine, double tol);
 
 /* Solve every cell on `stream` (a hipStream_t, or NULL for the default stream).  dA, dB, dX,
@@ -114,6 +114,13 @@ int hipla_adjoint (hipla_plan plan, const double *dX, const double *dG, double *
  * Runs synchronously on `stream`.  dA..dX must be valid inputs as for hipla_solve. */
 int hipla_plan_autotune (hipla_plan plan, const double *dA, const double *dB, double *dX, int *dcodes,
                          void *stream);
+
+/* Profiling helper (no reference counterpart): launch each size-class group of the plan alone
+ * `reps` times on the plan's own stream, bracketed by HIP events, and write the average ms per
+ * launch into ms_out[0 .. ngroups-1]; ms_out[ngroups] = the FP64 re-solve pass of a mixed plan
+ * (0 for FP64 plans), ms_out[ngroups+1] = a whole hipla_solve.  nout >= ngroups + 2. */
+int hipla_plan_time_groups (hipla_plan plan, const double *dA, const double *dB, double *dX, int *dcodes,
+                            hipla_cell_stats *dstats, int reps, double *ms_out, int nout);
 
 /* Describe the plan's schedule (size classes, kernel variant per class, tiles) into buf. */
 int hipla_plan_describe (hipla_plan plan, char *buf, int buflen);

@@ -276,15 +276,17 @@ ref_lu_decomp (double *a, int n, size_t *perm, unsigned *ipiv, int *signum)
   return 0;
 }
 
-/* ---- independent restatement: plain right-looking GEPP with the same update rule ----
+/* ---- independent restatement: plain right-looking GEPP (the GPU kernels' order) ----
  *
  * Every element a(pos, j) receives its updates for k = 0, 1, ... in ascending order, each as
- * a - l*u with the product rounded first, exactly as in the recursion above.  The recursion's
- * only other effect is where dgemm (cblas/source_gemm_r.h:76) skips a zero multiplier while
- * dger / dtrsm do not; that changes signs of zeros and inf*0 only.  ref_skip_window() gives,
- * for step k and row position pos, the column window [lo, hi) in which the recursion reaches
- * a(pos, j) through dgemm: lo = end of k's leaf panel, hi = the first split boundary on k's
- * path that lies above pos.  The GPU kernels use the same rule. */
+ * a - l*u with the product rounded first, exactly as in the recursion above, so the values
+ * agree with the recursion.  The recursion's dgemm (cblas/source_gemm_r.h:76) additionally skips
+ * exactly-zero multipliers while its dger / dtrsm do not; that decides only signs of zero results
+ * and whether inf*0 produces a NaN.  This restatement (like the GPU kernels) always applies the
+ * update.  ref_skip_window() gives, for step k and row position pos, the column window [lo, hi)
+ * in which the recursion would reach a(pos, j) through dgemm (lo = end of k's leaf panel, hi =
+ * the first split boundary on k's path above pos); it documents the rule and is exercised by the
+ * tests, but no kernel uses it. */
 void
 ref_skip_window (int n, int k, int pos, int *lo, int *hi)
 {
@@ -341,12 +343,8 @@ ref_lu_decomp_rightlooking (double *a, int n, size_t *perm, unsigned *ipiv, int 
     for (int i = k + 1; i < n; i++)
     {
       const double l = AT (A, i, k);
-      int lo, hi;
-      ref_skip_window (n, k, i, &lo, &hi);
       for (int j = k + 1; j < n; j++)
       {
-        if (l == 0.0 && j >= lo && j < hi)
-          continue;
         const double prod = l * AT (A, k, j);
         AT (A, i, j) = AT (A, i, j) - prod;
       }

@@ -28,7 +28,15 @@ struct SolveArgs
   // GSL ipiv (row swapped with step k) at piv + offV[c]
   double *LU;
   int *piv;
+  // optional device-side list length (FP64 re-solve of cells the mixed solver hands back):
+  // the kernels then process min(*dcount, nlist) cells, persistent over a fixed grid
+  const int *dcount;
 };
+
+__device__ __forceinline__ int list_len (const SolveArgs &s)
+{
+  return s.dcount ? min (*s.dcount, s.nlist) : s.nlist;
+}
 
 // ----------------------------------------------------------------------------------------
 // Pivot key.  GSL's idamax (cblas/source_iamax_r.h:30-35) starts from max = 0 at the first
@@ -215,4 +223,82 @@ __device__ __forceinline__ double wave_bcast_f64 (double v, int src)
   int lo = __builtin_amdgcn_readlane (__double2loint (v), src);
   int hi = __builtin_amdgcn_readlane (__double2hiint (v), src);
   return __hiloint2double (hi, lo);
+}
+
+// ----------------------------------------------------------------------------------------
+// DPP wave reductions (no LDS traffic).  quad_perm [1,0,3,2] and [2,3,0,1], row_half_mirror and
+// row_mirror leave every lane of a 16-lane row holding the row's result; the four rows are then
+// combined from lanes 0, 16, 32, 48 with readlane, so the result is wave-uniform.
+// ----------------------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32 (int v)
+{
+  return __builtin_amdgcn_update_dpp (0, v, CTRL, 0xF, 0xF, false);
+}
+
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp_max_u64_step (unsigned long long v)
+{
+  const int lo = dpp_i32<CTRL> ((int) (unsigned) v);
+  const int hi = dpp_i32<CTRL> ((int) (unsigned) (v >> 32));
+  const unsigned long long o = ((unsigned long long) (unsigned) hi << 32) | (unsigned) lo;
+  return o > v ? o : v;
+}
+
+__device__ __forceinline__ unsigned long long rl_u64 (unsigned long long v, int lane)
+{
+  const unsigned lo = (unsigned) __builtin_amdgcn_readlane ((int) (unsigned) v, lane);
+  const unsigned hi = (unsigned) __builtin_amdgcn_readlane ((int) (unsigned) (v >> 32), lane);
+  return ((unsigned long long) hi << 32) | lo;
+}
+
+// Wave-uniform maximum of an unsigned 64-bit key.
+__device__ __forceinline__ unsigned long long wave_max_u64_dpp (unsigned long long v)
+{
+  v = dpp_max_u64_step<0xB1> (v);
+  v = dpp_max_u64_step<0x4E> (v);
+  v = dpp_max_u64_step<0x141> (v);
+  v = dpp_max_u64_step<0x140> (v);
+  unsigned long long r = rl_u64 (v, 0);
+  const unsigned long long r1 = rl_u64 (v, 16), r2 = rl_u64 (v, 32), r3 = rl_u64 (v, 48);
+  r = r1 > r ? r1 : r;
+  r = r2 > r ? r2 : r;
+  r = r3 > r ? r3 : r;
+  return r;
+}
+
+// Wave-uniform minimum of a 32-bit int.
+__device__ __forceinline__ int wave_min_i32_dpp (int v)
+{
+  v = min (v, dpp_i32<0xB1> (v));
+  v = min (v, dpp_i32<0x4E> (v));
+  v = min (v, dpp_i32<0x141> (v));
+  v = min (v, dpp_i32<0x140> (v));
+  int r = __builtin_amdgcn_readlane (v, 0);
+  r = min (r, __builtin_amdgcn_readlane (v, 16));
+  r = min (r, __builtin_amdgcn_readlane (v, 32));
+  r = min (r, __builtin_amdgcn_readlane (v, 48));
+  return r;
+}
+
+// GSL pivot choice within a wave: among lanes with `alive`, the largest pivot_key, ties to the
+// smallest `pos`.  Returns the winning lane (uniform), or -1 when no lane is alive.
+__device__ __forceinline__ int wave_pivot_lane (bool alive, long long key, int pos)
+{
+  const unsigned long long k1 = alive ? (unsigned long long) key + 1ull : 0ull;
+  const unsigned long long mk = wave_max_u64_dpp (k1);
+  if (mk == 0ull)
+    return -1;
+  unsigned long long cand = __ballot (k1 == mk);
+  if (__popcll (cand) == 1)
+    return __ffsll ((long long) cand) - 1;
+  const int mp = wave_min_i32_dpp ((k1 == mk) ? pos : 0x7fffffff);
+  cand = __ballot (k1 == mk && pos == mp);
+  return __ffsll ((long long) cand) - 1;
+}
+
+// Uniform read of a double held by lane `src`.
+__device__ __forceinline__ double rl_f64 (double v, int src)
+{
+  return __longlong_as_double ((long long) rl_u64 ((unsigned long long) __double_as_longlong (v), src));
 }

@@ -7,14 +7,16 @@
 
 #include "hipla_dev.h"
 
-#define BLK_NB 16   // panel width of the blocked FP64 kernel
+#define BLK_NB 8   // panel width of the blocked FP64 kernel
+#define BLK_MAXM 16 // blocked FP64 kernel: n <= 64 * BLK_MAXM (1024)
 
 // ---- FP64, GSL operation order (k_solve_f64.hip) ----
 size_t wave_f64_smem (int nc, int wpb, bool keep);
 size_t lds_f64_smem (int n);
 size_t blocked_f64_smem (int nmax);
-hipError_t launch_wave_f64 (const SolveArgs &a, int nc, bool keep, hipStream_t st);
-hipError_t launch_lds_f64 (const SolveArgs &a, int nmax, bool keep, hipStream_t st);
+hipError_t launch_wave_f64 (const SolveArgs &a, int nc, bool keep, int grid, hipStream_t st);
+hipError_t launch_lds_f64 (const SolveArgs &a, int nmax, bool keep, int grid, hipStream_t st);
 hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, double *work, int64_t wstride, bool keep,
                                hipStream_t st);
 hipError_t setup_solve_f64_attrs (size_t max_smem);
+int lds_f64_limit ();   // largest n the LDS-resident FP64 kernel holds on this device

@@ -53,6 +53,8 @@ struct Group
   int count = 0;
   int grid = 0;       // persistent grid size (blocked kernels)
   int stream = 0;     // fork-join branch
+  int direct = 0;     // mixed plans: 1 = the tuner routes this size class straight to FP64
+  double t_mixed = -1.0, t_direct = -1.0;   // tuner measurements (ms), -1 if not tuned
 };
 
 struct hipla_plan_s
@@ -61,7 +63,7 @@ struct hipla_plan_s
   int mode = 0;
   int flags = 0;
   int max_refine = 5;
-  double tol = 1e-13;
+  double tol = 1e-12;
   std::vector<int> n;
   std::vector<int64_t> offA, offV;
   int64_t totA = 0, totV = 0;
@@ -75,9 +77,8 @@ struct hipla_plan_s
   double *d_work = nullptr;   // blocked kernels: per-workgroup slot
   int64_t wstride = 0;
   int work_slots = 0;
-  float *d_work32 = nullptr;  // mixed blocked kernels
-  int64_t wstride32 = 0;
-  int *d_fallback = nullptr;  // mixed: [0] = count, [1..] = cells to re-solve in FP64
+  int *d_fallback = nullptr;  // mixed: 5 lists [count, cells...] of cells to re-solve in FP64
+  int fb_route_used[5] = { 0, 0, 0, 0, 0 };
   double *d_LU = nullptr;     // keep-factors store (A's layout)
   int *d_piv = nullptr;       // keep-factors pivots (B's layout)
   // fork-join streams / events
@@ -96,8 +97,16 @@ struct hipla_plan_s
 // launch every group of a solve / invert on `st` (fork-join over aux streams)
 int plan_enqueue_solve (hipla_plan p, const double *dA, const double *dB, double *dX, int *dcodes,
                         hipla_cell_stats *dstats, hipStream_t st);
+int plan_enqueue_mixed (hipla_plan p, const double *dA, const double *dB, double *dX, int *dcodes,
+                        hipla_cell_stats *dstats, hipStream_t st);
 int plan_enqueue_invert (hipla_plan p, const double *dA, double *dInv, int *dcodes, hipla_cell_stats *dstats,
                          hipStream_t st);
+
+// mixed plans: one group alone, the fallback-counter reset, the FP64 re-solve pass
+int enqueue_mixed_group (hipla_plan p, int gi, const double *dA, const double *dB, double *dX, int *dcodes,
+                         hipla_cell_stats *dstats, hipStream_t s);
+int enqueue_mixed_reset (hipla_plan p, hipStream_t st);
+int enqueue_mixed_fallback (hipla_plan p, const double *dA, const double *dB, double *dX, int *dcodes, hipStream_t st);
 
 // kernel-family attribute setup (dynamic LDS above 64 KB)
 hipError_t setup_mixed_attrs (size_t max_smem);

@@ -32,22 +32,12 @@ static const char *kind_name (int k)
   case KK_WAVE_MIX:
     return "wave_mixed";
   case KK_LDS_MIX:
-    return "lds_mixed";
+    return "blk_mixed";
   case KK_BLOCKED_MIX:
-    return "blocked_mixed";
+    return "fp64_over_256";
   default:
     return "?";
   }
-}
-
-// Largest order the LDS-resident FP64 kernel can hold ([A|b] plus two work vectors).
-static int lds_f64_limit ()
-{
-  const size_t cap = hipla_device_info ().max_smem - 2048;
-  int n = 1;
-  while (lds_f64_smem (n + 1) <= cap)
-    n++;
-  return n;
 }
 
 static void route_fp64 (int n, int &kind, int &param)
@@ -87,7 +77,7 @@ static int free_plan_device (hipla_plan p)
     (void) hipGraphDestroy (p->graph);
   p->gexec = nullptr;
   p->graph = nullptr;
-  void *bufs[] = { p->d_n, p->d_offA, p->d_offV, p->d_list, p->d_work, p->d_work32, p->d_fallback, p->d_LU, p->d_piv };
+  void *bufs[] = { p->d_n, p->d_offA, p->d_offV, p->d_list, p->d_work, p->d_fallback, p->d_LU, p->d_piv };
   for (void *b : bufs)
     if (b)
       (void) hipFree (b);
@@ -166,8 +156,8 @@ extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, con
   });
   p->list = idx;
   const DeviceInfo &dev = hipla_device_info ();
-  int64_t wneed = 0, wneed32 = 0;
-  int wslots = 0, wslots32 = 0;
+  int64_t wneed = 0;
+  int wslots = 0;
   for (int i = 0; i < ncells;)
   {
     int j = i;
@@ -180,20 +170,39 @@ extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, con
     g.count = j - i;
     g.nmax = n[idx[i]];
     g.stream = (int) p->groups.size ();
-    if (g.kind == KK_BLOCKED_F64)
+    if (g.kind == KK_BLOCKED_F64 || g.kind == KK_BLOCKED_MIX)
     {
       g.grid = std::min (g.count, 2 * dev.num_cu);
       wneed = std::max<int64_t> (wneed, (int64_t) g.nmax * g.nmax + g.nmax);
       wslots = std::max (wslots, g.grid);
     }
-    if (g.kind == KK_BLOCKED_MIX)
-    {
-      g.grid = std::min (g.count, 2 * dev.num_cu);
-      wneed32 = std::max<int64_t> (wneed32, blocked_mixed_ws_floats (g.nmax));
-      wslots32 = std::max (wslots32, g.grid);
-    }
     p->groups.push_back (g);
     i = j;
+  }
+
+  // mixed mode: FP64 routes the fallback re-solve can need (the blocked one needs a workspace)
+  if (mode == HIPLA_MODE_MIXED)
+  {
+    int nblk = 0, nbmax = 0;
+    for (int c = 0; c < ncells; c++)
+    {
+      if (kind[c] == KK_BLOCKED_MIX)
+        continue;
+      int fk, fp;
+      route_fp64 (n[c], fk, fp);
+      const int r = fk == KK_WAVE_F64 ? (fp == 16 ? 0 : fp == 32 ? 1 : 2) : fk == KK_LDS_F64 ? 3 : 4;
+      p->fb_route_used[r] = 1;
+      if (r == 4)
+      {
+        nblk++;
+        nbmax = std::max (nbmax, n[c]);
+      }
+    }
+    if (nblk)
+    {
+      wneed = std::max<int64_t> (wneed, (int64_t) nbmax * nbmax + nbmax);
+      wslots = std::max (wslots, std::min (nblk, dev.num_cu));
+    }
   }
 
   // ---- device copies and workspaces ----
@@ -223,13 +232,8 @@ extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, con
     p->work_slots = wslots;
     PCHECK (hipMalloc (&p->d_work, sizeof (double) * p->wstride * wslots));
   }
-  if (wslots32 > 0)
-  {
-    p->wstride32 = (wneed32 + 63) & ~(int64_t) 63;
-    PCHECK (hipMalloc (&p->d_work32, sizeof (float) * p->wstride32 * wslots32));
-  }
   if (mode == HIPLA_MODE_MIXED)
-    PCHECK (hipMalloc (&p->d_fallback, sizeof (int) * (ncells + 1)));
+    PCHECK (hipMalloc (&p->d_fallback, sizeof (int) * 5 * (ncells + 1)));
   if (flags & HIPLA_FLAG_KEEP_FACTORS)
   {
     PCHECK (hipMalloc (&p->d_LU, sizeof (double) * p->totA));
@@ -272,8 +276,20 @@ extern "C" int hipla_plan_describe (hipla_plan p, char *buf, int buflen)
   for (size_t i = 0; i < p->groups.size (); i++)
   {
     const Group &g = p->groups[i];
-    snprintf (line, sizeof line, "%s{\"kernel\": \"%s\", \"param\": %d, \"nmax\": %d, \"count\": %d, \"grid\": %d}",
-              i ? ", " : "", kind_name (g.kind), g.param, g.nmax, g.count, g.grid);
+    // algorithmic work of the group's solves: bytes 8 (n^2 + 2 n), flops 2/3 n^3 + 2 n^2
+    double bytes = 0.0, flops = 0.0;
+    for (int j = g.first; j < g.first + g.count; j++)
+    {
+      const double nn = p->n[p->list[j]];
+      bytes += 8.0 * (nn * nn + 2.0 * nn);
+      flops += 2.0 / 3.0 * nn * nn * nn + 2.0 * nn * nn;
+    }
+    snprintf (line, sizeof line,
+              "%s{\"kernel\": \"%s\", \"param\": %d, \"nmax\": %d, \"count\": %d, \"grid\": %d, "
+              "\"direct_fp64\": %d, \"t_mixed_ms\": %.4f, \"t_direct_ms\": %.4f, \"bytes\": %.0f, "
+              "\"flops\": %.0f}",
+              i ? ", " : "", kind_name (g.kind), g.param, g.nmax, g.count, g.grid, g.direct, g.t_mixed, g.t_direct,
+              bytes, flops);
     s += line;
   }
   s += "]}";
@@ -299,6 +315,7 @@ static SolveArgs make_args (hipla_plan p, const double *dA, const double *dB, do
   a.nlist = g.count;
   a.LU = p->d_LU;
   a.piv = p->d_piv;
+  a.dcount = nullptr;
   return a;
 }
 
@@ -308,14 +325,23 @@ static hipError_t launch_group_f64 (hipla_plan p, const SolveArgs &a, const Grou
   switch (g.kind)
   {
   case KK_WAVE_F64:
-    return launch_wave_f64 (a, g.param, keep, st);
+    return launch_wave_f64 (a, g.param, keep, 0, st);
   case KK_LDS_F64:
-    return launch_lds_f64 (a, g.nmax, keep, st);
+    return launch_lds_f64 (a, g.nmax, keep, 0, st);
   case KK_BLOCKED_F64:
     return launch_blocked_f64 (a, g.nmax, std::min (g.grid, std::max (a.nlist, 1)), p->d_work, p->wstride, keep, st);
   default:
     return hipErrorInvalidValue;
   }
+}
+
+int enqueue_fp64_group (hipla_plan p, int gi, const double *dA, const double *dB, double *dX, int *dcodes,
+                        hipStream_t st)
+{
+  const Group &g = p->groups[gi];
+  SolveArgs a = make_args (p, dA, dB, dX, dcodes, g);
+  HIPLA_CHECK (launch_group_f64 (p, a, g, st));
+  return 0;
 }
 
 int plan_enqueue_solve (hipla_plan p, const double *dA, const double *dB, double *dX, int *dcodes,

@@ -16,7 +16,3 @@ extern "C" int hipla_adjoint (hipla_plan, const double *, const double *, double
   hipla_set_detail ("adjoint not available");
   return 5;
 }
-extern "C" int hipla_plan_autotune (hipla_plan, const double *, const double *, double *, int *, void *)
-{
-  return 0;
-}

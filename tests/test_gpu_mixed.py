@@ -1,0 +1,94 @@
+"""GPU checks of the mixed-precision solver (equilibrated FP32 LU + FP64 refinement + FP64 fallback).
+
+Tolerance (stated by the north-star contract): per-cell normwise relative error
+||x - x_oracle||_inf / ||x_oracle||_inf <= 1e-8 after refinement, where the oracle is the FP64
+GSL-order solve.  Cells the solver hands back to FP64 must then equal the oracle exactly.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-8
+
+
+def _relerr(batch, X, Xo, codes_oracle=None):
+    """Per-cell normwise relative error; 0 for cells the oracle reports singular (code 4: x is
+    not written by either side)."""
+    errs = np.zeros(batch.ncells)
+    for c in range(batch.ncells):
+        if codes_oracle is not None and codes_oracle[c] == 4:
+            continue
+        o, n = batch.offV[c], batch.n[c]
+        xo = Xo[o:o + n]
+        errs[c] = np.abs(X[o:o + n] - xo).max() / max(np.abs(xo).max(), 1e-300)
+    return errs
+
+
+def _cond_est(batch, cells):
+    out = {}
+    for c in cells:
+        a, _ = batch.cell(c)
+        out[c] = np.linalg.cond(a)
+    return out
+
+
+@pytest.mark.parametrize("scale", [0.02])
+def test_mixed_uneven_matches_oracle(hip, oracle_lib, scale):
+    from python_amd.generators import uneven
+    batch = uneven(scale=scale)
+    print("cells", batch.ncells, flush=True)
+    X, codes, stats = hip.solve_batch(batch, mode="mixed")
+    Xo, co = oracle_lib.solve_batch(batch.n, batch.offA, batch.offV, batch.A, batch.B, nthreads=8)
+    err = _relerr(batch, X, Xo, co)
+    assert np.array_equal(codes == 4, co == 4)
+    fb = stats["fallback"]
+    print("max err", err.max(), "fallback", int((fb == 1).sum()), "steps hist",
+          np.bincount(stats["refine_steps"][fb == 0]), flush=True)
+    bad = np.where(err > TOL)[0]
+    if len(bad):
+        ce = _cond_est(batch, bad[:10])
+        print("over tol:", [(int(c), int(batch.n[c]), float(err[c]), float(ce[c])) for c in bad[:10]], flush=True)
+    assert np.all(err <= TOL)
+    # fallback cells are the FP64 GSL-order answer
+    for c in np.where((fb == 1) & (co != 4))[0]:
+        o, n = batch.offV[c], batch.n[c]
+        assert np.array_equal(X[o:o + n], Xo[o:o + n])
+    assert np.array_equal(codes, co) or np.mean(codes == co) > 0.99
+
+
+def test_mixed_sizes_sweep(hip, oracle_lib):
+    from python_amd.generators import ion_matrices, pack
+    rng = np.random.default_rng(17)
+    mats, rhs = [], []
+    for n in (1, 2, 5, 12, 16, 17, 30, 32, 33, 63, 64, 65, 80, 96, 97, 128, 129, 150, 192, 193, 200, 240, 256):
+        A, b = ion_matrices(rng, 2, n)
+        mats += [A[0], A[1]]
+        rhs += [b[0], b[1]]
+        D = rng.standard_normal((n, n)) + 2 * np.sqrt(n) * np.eye(n)
+        mats.append(D)
+        rhs.append(rng.standard_normal(n))
+    batch = pack(mats, rhs)
+    X, codes, stats = hip.solve_batch(batch, mode="mixed")
+    Xo, co = oracle_lib.solve_batch(batch.n, batch.offA, batch.offV, batch.A, batch.B, nthreads=8)
+    err = _relerr(batch, X, Xo, co)
+    assert np.array_equal(codes == 4, co == 4)
+    for c in range(batch.ncells):
+        print(int(batch.n[c]), float(err[c]), int(stats["refine_steps"][c]), int(stats["fallback"][c]),
+              float(stats["backward_err"][c]), flush=True) if err[c] > TOL else None
+    assert np.all(err <= TOL)
+
+
+def test_mixed_singular_goes_to_fp64(hip, oracle_lib):
+    from python_amd.generators import pack
+    rng = np.random.default_rng(2)
+    mats, rhs = [], []
+    for n in (10, 40, 100, 200):
+        A = rng.standard_normal((n, n))
+        A[:, 3] = 0.0
+        mats.append(A)
+        rhs.append(rng.standard_normal(n))
+    batch = pack(mats, rhs)
+    X, codes, stats = hip.solve_batch(batch, mode="mixed")
+    assert np.all(codes == 4)
+    assert np.all(stats["fallback"] == 1)
