@@ -1,0 +1,113 @@
+"""Differentiable batched solve and d(populations)/dT_e for the ionization balance.
+
+``SolveFn`` is a ``torch.autograd.Function`` over a batched FP64 solve (GSL operation order):
+forward solves A x = b per cell and keeps the LU factors in the plan; backward calls the adjoint
+kernel (hipla_adjoint), which reuses those factors to solve A^T lambda = dL/dx and returns
+dL/dA = -lambda x^T and dL/db = lambda.  Gradients with respect to anything A and b are built
+from (the electron temperature, below) then follow from torch's own chain rule.
+
+``IonTe`` assembles the temperature-dependent ionization matrices of the adjoint workload in
+torch: ionization i -> i+1 at C T^0.5 exp(-E / kT), recombination i+1 -> i at
+alpha0 (T / 1e4)^-0.7, diagonals minus the column sums, and the abundance rows that
+``populate_ion_rate_matrix`` places on each element's first ion (source/matrix_ion.c:633-655).
+The reference obtains T_e from a root find around the solve (``calc_te``, source/ionization.c:486);
+``newton_te`` is one Newton step on T_e per cell driven by the adjoint gradient.
+
+A plan keeps the factors of its latest solve only: run backward before the next forward on the
+same plan.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .generators import KB_EV
+from .solver import Plan
+
+
+class SolveFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, A, B, plan):
+        X, codes, _ = plan.solve(A.detach().contiguous(), B.detach().contiguous())
+        ctx.plan = plan
+        ctx.save_for_backward(X)
+        ctx.mark_non_differentiable(codes)
+        return X, codes
+
+    @staticmethod
+    def backward(ctx, gX, gcodes):
+        (X,) = ctx.saved_tensors
+        plan = ctx.plan
+        Lam = torch.empty_like(X)
+        need_A = ctx.needs_input_grad[0]
+        GradA = torch.empty(plan.totA, dtype=X.dtype, device=X.device) if need_A else None
+        plan.adjoint(X, gX.contiguous(), Lam, GradA)
+        return GradA, Lam, None
+
+
+def solve(A, B, plan: Plan):
+    """x = A^-1 b per cell (flat ragged layout of ``plan``), differentiable in A and b."""
+    X, codes = SolveFn.apply(A, B, plan)
+    return X, codes
+
+
+class IonTe:
+    """Ionization matrices of order n whose rates depend on T_e (one parameter set per cell)."""
+
+    def __init__(self, te_batch: dict, device="cuda"):
+        self.n = te_batch["A"].shape[1]
+        self.ncells = te_batch["A"].shape[0]
+        self.blocks = te_batch["blocks"]
+        self.device = device
+        self.terms = []
+        for (s, m, ab, C, E, a0) in te_batch["params"]:
+            if m > 1:
+                t = lambda x: torch.as_tensor(x, dtype=torch.float64, device=device)
+                self.terms.append((s, m, t(C), t(E), t(a0)))
+        self.cons = [(s, m, ab) for (s, m, ab, *_r) in te_batch["params"]]
+        self.offA = np.arange(self.ncells, dtype=np.int64) * self.n * self.n
+        self.offV = np.arange(self.ncells, dtype=np.int64) * self.n
+
+    def assemble(self, T):
+        """A(T) [ncells, n, n] and b [ncells, n] as differentiable torch tensors."""
+        n, nc = self.n, self.ncells
+        A = torch.zeros(nc, n, n, dtype=torch.float64, device=self.device)
+        Tc = T[:, None]
+        for (s, m, C, E, a0) in self.terms:
+            k = torch.arange(m - 1, device=self.device)
+            ion = C * torch.sqrt(Tc) * torch.exp(-E / (KB_EV * Tc))
+            rec = a0 * (Tc / 1e4) ** -0.7
+            A = A.index_put((torch.arange(nc, device=self.device)[:, None], s + k + 1, s + k), ion)
+            A = A.index_put((torch.arange(nc, device=self.device)[:, None], s + k, s + k + 1), rec)
+        d = torch.arange(n, device=self.device)
+        A = A - torch.diag_embed(A.sum(dim=1))
+        b = torch.zeros(nc, n, dtype=torch.float64, device=self.device)
+        keep = torch.ones(n, dtype=torch.float64, device=self.device)
+        ones = torch.zeros(n, n, dtype=torch.float64, device=self.device)
+        for (s, m, ab) in self.cons:
+            keep[s] = 0.0
+            ones[s, s:s + m] = 1.0
+            b[:, s] = ab
+        A = A * keep[None, :, None] + ones[None]
+        return A, b
+
+    def plan(self):
+        return Plan(np.full(self.ncells, self.n, dtype=np.int32), self.offA, self.offV, mode="fp64", graph=False,
+                    keep_factors=True)
+
+
+def loss_and_grad(model: IonTe, plan: Plan, T, w):
+    """Cooling proxy L_c = sum_i w_ci x_ci per cell, and dL_c/dT_c through the adjoint."""
+    T = T.detach().clone().requires_grad_(True)
+    A, b = model.assemble(T)
+    X, codes = solve(A.reshape(-1), b.reshape(-1), plan)
+    L = (X.view(model.ncells, model.n) * w).sum(dim=1)
+    (g,) = torch.autograd.grad(L.sum(), T)
+    return L.detach(), g.detach(), codes
+
+
+def newton_te(model: IonTe, plan: Plan, T0, w, L_target):
+    """One Newton step on T_e per cell toward L(T) = L_target."""
+    L0, dL, codes = loss_and_grad(model, plan, T0, w)
+    step = (L0 - L_target) / dL
+    return T0 - step, L0, dL

@@ -8,6 +8,7 @@
 #include "hipla_dev.h"
 
 #define BLK_NB 8   // panel width of the blocked FP64 kernel
+#define INV_NB 8    // panel width / backward block of the inverse kernel
 #define BLK_MAXM 16 // blocked FP64 kernel: n <= 64 * BLK_MAXM (1024)
 
 // ---- FP64, GSL operation order (k_solve_f64.hip) ----

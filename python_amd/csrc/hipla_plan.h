@@ -79,6 +79,10 @@ struct hipla_plan_s
   int work_slots = 0;
   int *d_fallback = nullptr;  // mixed: 5 lists [count, cells...] of cells to re-solve in FP64
   int fb_route_used[5] = { 0, 0, 0, 0, 0 };
+  int *d_list_desc = nullptr;  // every cell, largest order first (inverse launches)
+  double *d_inv_work = nullptr; // inverse: per-workgroup LU workspace
+  int64_t inv_wstride = 0;
+  int inv_slots = 0;
   double *d_LU = nullptr;     // keep-factors store (A's layout)
   int *d_piv = nullptr;       // keep-factors pivots (B's layout)
   // fork-join streams / events
@@ -101,6 +105,7 @@ int plan_enqueue_mixed (hipla_plan p, const double *dA, const double *dB, double
                         hipla_cell_stats *dstats, hipStream_t st);
 int plan_enqueue_invert (hipla_plan p, const double *dA, double *dInv, int *dcodes, hipla_cell_stats *dstats,
                          hipStream_t st);
+int plan_invert_prepare (hipla_plan p);   // allocates the inverse workspace (outside any capture)
 
 // mixed plans: one group alone, the fallback-counter reset, the FP64 re-solve pass
 int enqueue_mixed_group (hipla_plan p, int gi, const double *dA, const double *dB, double *dX, int *dcodes,

@@ -77,7 +77,7 @@ static int free_plan_device (hipla_plan p)
     (void) hipGraphDestroy (p->graph);
   p->gexec = nullptr;
   p->graph = nullptr;
-  void *bufs[] = { p->d_n, p->d_offA, p->d_offV, p->d_list, p->d_work, p->d_fallback, p->d_LU, p->d_piv };
+  void *bufs[] = { p->d_n, p->d_offA, p->d_offV, p->d_list, p->d_list_desc, p->d_inv_work, p->d_work, p->d_fallback, p->d_LU, p->d_piv };
   for (void *b : bufs)
     if (b)
       (void) hipFree (b);
@@ -226,6 +226,13 @@ extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, con
   PCHECK (hipMemcpy (p->d_offA, offA, sizeof (int64_t) * ncells, hipMemcpyHostToDevice));
   PCHECK (hipMemcpy (p->d_offV, offV, sizeof (int64_t) * ncells, hipMemcpyHostToDevice));
   PCHECK (hipMemcpy (p->d_list, p->list.data (), sizeof (int) * ncells, hipMemcpyHostToDevice));
+  {
+    std::vector<int> desc (ncells);
+    std::iota (desc.begin (), desc.end (), 0);
+    std::stable_sort (desc.begin (), desc.end (), [&] (int a, int b) { return n[a] > n[b]; });
+    PCHECK (hipMalloc (&p->d_list_desc, sizeof (int) * ncells));
+    PCHECK (hipMemcpy (p->d_list_desc, desc.data (), sizeof (int) * ncells, hipMemcpyHostToDevice));
+  }
   if (wslots > 0)
   {
     p->wstride = (wneed + 63) & ~(int64_t) 63;
@@ -468,5 +475,8 @@ extern "C" int hipla_invert (hipla_plan p, const double *dA, double *dInv, int *
     hipla_set_detail ("hipla_invert: bad argument");
     return 5;
   }
+  int rc = plan_invert_prepare (p);
+  if (rc)
+    return rc;
   return run_maybe_graph (p, 1, dA, dInv, dcodes, dstats, nullptr, (hipStream_t) stream, enq_invert);
 }

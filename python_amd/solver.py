@@ -205,11 +205,14 @@ def to_device(batch, device="cuda"):
 
 
 def solve_batch(batch, mode: str = "fp64", graph: bool = True, device="cuda", **kw):
-    """Solve a generators.RaggedBatch on the GPU; returns numpy (X, codes, stats-or-None)."""
+    """Solve a generators.RaggedBatch on the GPU; returns numpy (X, codes, stats-or-None).
+
+    X starts as zeros, so the solution of a singular cell (code 4, x not written) reads 0.
+    """
     import torch
     plan = Plan.for_batch(batch, mode=mode, graph=graph, **kw)
     A, B = to_device(batch, device)
-    X, codes, stats = plan.solve(A, B, stats=(mode == "mixed"))
+    X, codes, stats = plan.solve(A, B, torch.zeros_like(B), stats=(mode == "mixed"))
     torch.cuda.synchronize()
     out = (X.cpu().numpy(), codes.cpu().numpy(), stats_to_numpy(stats) if stats is not None else None)
     plan.close()

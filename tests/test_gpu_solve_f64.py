@@ -60,13 +60,22 @@ def test_batched_random_sizes_equal_oracle(hip, oracle_lib):
             assert _eq(xs, xo), (c, a.shape[0], np.abs(xs - xo).max())
 
 
+def _eq_nonsingular(batch, X, Xo, co):
+    """Equal values in every cell the oracle solved (a code-4 cell's x is not written)."""
+    for c in np.where(co != 4)[0]:
+        o, n = batch.offV[c], batch.n[c]
+        if not _eq(X[o:o + n], Xo[o:o + n]):
+            return False
+    return True
+
+
 def test_plain_small_900(hip, oracle_lib):
     from python_amd.generators import plain_small
     batch = plain_small()
     X, codes, _ = hip.solve_batch(batch, mode="fp64")
     Xo, co = oracle_lib.solve_batch(batch.n, batch.offA, batch.offV, batch.A, batch.B, nthreads=8)
     assert np.array_equal(codes, co)
-    assert _eq(X, Xo)
+    assert _eq_nonsingular(batch, X, Xo, co)
 
 
 def test_uneven_subset_equal_oracle(hip, oracle_lib):
@@ -74,8 +83,9 @@ def test_uneven_subset_equal_oracle(hip, oracle_lib):
     batch = uneven(scale=0.1)       # 1,000 cells with the full size mix
     X, codes, _ = hip.solve_batch(batch, mode="fp64")
     Xo, co = oracle_lib.solve_batch(batch.n, batch.offA, batch.offV, batch.A, batch.B, nthreads=8)
+    print("singular (code 4) cells", int((co == 4).sum()), flush=True)
     assert np.array_equal(codes, co)
-    assert _eq(X, Xo)
+    assert _eq_nonsingular(batch, X, Xo, co)
 
 
 def test_codes_singular_and_residual(hip, oracle_lib):
