@@ -297,6 +297,23 @@ __device__ __forceinline__ int wave_pivot_lane (bool alive, long long key, int p
   return __ffsll ((long long) cand) - 1;
 }
 
+// Combine of per-wave pivot candidates (key -1: none; ties to the smallest position): lane w
+// reads candidate w and the wave reduces them, so the combine costs one LDS read per lane and
+// one DPP reduction instead of a chain of dependent reads.  Returns the winner's position and
+// row (wave-uniform); nw <= 64.
+__device__ __forceinline__ void combine_pivot_candidates (const long long *rkey, const int *rpos, const int *rrow,
+                                                          int nw, int lane, int &pos, int &row)
+{
+  const bool has = lane < nw;
+  const long long key = has ? rkey[lane] : -1;
+  const int q = has ? rpos[lane] : 0x7fffffff;
+  const int r = has ? rrow[lane] : 0;
+  const int wl = wave_pivot_lane (key >= 0, key, q);
+  const int src = wl >= 0 ? wl : 0;
+  pos = __builtin_amdgcn_readlane (q, src);
+  row = __builtin_amdgcn_readlane (r, src);
+}
+
 // Uniform read of a double held by lane `src`.
 __device__ __forceinline__ double rl_f64 (double v, int src)
 {

@@ -7,7 +7,8 @@
 
 #include "hipla_dev.h"
 
-#define BLK_NB 8   // panel width of the blocked FP64 kernel
+#define BLK_NB 16  // panel width of the blocked FP64 kernel (n <= 512; 8 above)
+#define BLK_BS 512 // threads per workgroup of the blocked FP64 kernel
 #define INV_NB 8    // panel width / backward block of the inverse kernel
 #define BLK_MAXM 16 // blocked FP64 kernel: n <= 64 * BLK_MAXM (1024)
 

@@ -52,6 +52,8 @@ struct Group
   int first = 0;      // offset into the plan's cell list
   int count = 0;
   int grid = 0;       // persistent grid size (blocked kernels)
+  int wslot0 = 0;     // first workspace slot of this group (groups that may run concurrently
+                      // with another blocked launch get disjoint slot ranges)
   int stream = 0;     // fork-join branch
   int direct = 0;     // mixed plans: 1 = the tuner routes this size class straight to FP64
   double t_mixed = -1.0, t_direct = -1.0;   // tuner measurements (ms), -1 if not tuned

@@ -157,7 +157,7 @@ extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, con
   p->list = idx;
   const DeviceInfo &dev = hipla_device_info ();
   int64_t wneed = 0;
-  int wslots = 0;
+  int wslots = 0, wcursor = 0;
   for (int i = 0; i < ncells;)
   {
     int j = i;
@@ -170,11 +170,16 @@ extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, con
     g.count = j - i;
     g.nmax = n[idx[i]];
     g.stream = (int) p->groups.size ();
-    if (g.kind == KK_BLOCKED_F64 || g.kind == KK_BLOCKED_MIX)
+    // groups that run (or, in mixed plans, may be routed to) the blocked FP64 kernel: the
+    // groups of one solve run concurrently, so each gets its own range of workspace slots
+    if (g.kind == KK_BLOCKED_F64 || g.kind == KK_BLOCKED_MIX ||
+        (mode == HIPLA_MODE_MIXED && g.nmax > lds_f64_limit ()))
     {
       g.grid = std::min (g.count, 2 * dev.num_cu);
+      g.wslot0 = wcursor;
+      wcursor += g.grid;
       wneed = std::max<int64_t> (wneed, (int64_t) g.nmax * g.nmax + g.nmax);
-      wslots = std::max (wslots, g.grid);
+      wslots = std::max (wslots, wcursor);
     }
     p->groups.push_back (g);
     i = j;
@@ -336,7 +341,8 @@ static hipError_t launch_group_f64 (hipla_plan p, const SolveArgs &a, const Grou
   case KK_LDS_F64:
     return launch_lds_f64 (a, g.nmax, keep, 0, st);
   case KK_BLOCKED_F64:
-    return launch_blocked_f64 (a, g.nmax, std::min (g.grid, std::max (a.nlist, 1)), p->d_work, p->wstride, keep, st);
+    return launch_blocked_f64 (a, g.nmax, std::min (g.grid, std::max (a.nlist, 1)),
+                               p->d_work + (size_t) g.wslot0 * p->wstride, p->wstride, keep, st);
   default:
     return hipErrorInvalidValue;
   }

@@ -930,14 +930,13 @@ static SolveArgs group_args (hipla_plan p, const double *dA, const double *dB, d
 static int launch_group_direct_fp64 (hipla_plan p, const SolveArgs &a, const Group &g, hipla_cell_stats *dstats,
                                      hipStream_t s)
 {
-  const DeviceInfo &dev = hipla_device_info ();
   if (g.nmax <= 64)
     HIPLA_CHECK (launch_wave_f64 (a, g.nmax <= 16 ? 16 : g.nmax <= 32 ? 32 : 64, false, 0, s));
   else if (g.nmax <= lds_f64_limit ())
     HIPLA_CHECK (launch_lds_f64 (a, g.nmax, false, 0, s));
   else
-    HIPLA_CHECK (launch_blocked_f64 (a, g.nmax, std::min ({ g.count, 2 * dev.num_cu, p->work_slots }), p->d_work,
-                                     p->wstride, false, s));
+    HIPLA_CHECK (launch_blocked_f64 (a, g.nmax, std::max (1, std::min (g.count, g.grid)),
+                                     p->d_work + (size_t) g.wslot0 * p->wstride, p->wstride, false, s));
   if (dstats)
   {
     hipLaunchKernelGGL (k_mark_fp64_stats, dim3 ((g.count + 255) / 256), dim3 (256), 0, s, a.list, g.count, dstats);

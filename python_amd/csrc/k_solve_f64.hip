@@ -17,13 +17,14 @@
 // differences).
 //
 // Three shapes, chosen per cell by the host plan:
-//   k_wave_f64<NC>     n <= NC <= 64: one wave per matrix, lane = row, row in registers, U rows
-//                      stashed in LDS as they are produced (no row swaps: positions tracked per
+//   k_wave_f64<NC>     n <= NC <= 64: one wave per matrix, lane = row, row in registers; a pivot
+//                      lane keeps its U row in registers (no row swaps: positions tracked per
 //                      lane, so GSL's tie rule uses positions, not lanes).
-//   k_lds_f64          n <= ~141: one workgroup per matrix, [A|b] resident in LDS.
+//   k_lds_f64          n <= ~141: one workgroup per matrix, [A|b] resident in LDS, positions
+//                      tracked instead of row swaps, one barrier per elimination step.
 //   k_blocked_f64<NB>  larger n: right-looking blocked LU, NB-column panel factored in LDS,
-//                      row swaps, U12 and the A22 update streamed through a per-workgroup
-//                      HBM workspace (L2 resident), persistent over the cell list.
+//                      row permutation, U12 and the A22 update streamed through a per-workgroup
+//                      HBM workspace, persistent over the cell list.
 #include "hipla_dev.h"
 #include "hipla_kernels.h"
 #include "hipla_plan.h"
@@ -33,65 +34,124 @@ static size_t hipla_device_max_smem ()
   return hipla_device_info ().max_smem;
 }
 
-// Back substitution U x = y in dtrsv order, run by one wave.  Row i of U is at U + i*ldu
-// (columns i..n-1 at their own column index), y_i at y[i*ys].  Lane l keeps x_{l + 64 m} in
-// xr[m]; for row i every lane forms its products U_ij x_j in parallel and the wave subtracts them
-// in ascending j through readlane (a chain of dependent FP64 subtractions, as GSL's).  Writes x to
-// xs (LDS) when xs is not null and returns with xr filled.
+// Back substitution U x = y in dtrsv order, run by one wave.  Row i of U is at U + r*ldu
+// (columns i..n-1 at their own column index) and y_i at y[r*ys], with r = rmap[i] when a row map
+// is given, else r = i.  Lane l keeps x_{l + 64 m} in xr[m].  For row i every lane forms its
+// products U_ij x_j in parallel and stages them in xs (n doubles of LDS); the wave then subtracts
+// them in ascending j from broadcast reads -- the chain of dependent FP64 subtractions of GSL's
+// dtrsv -- and divides by U_ii.  Row i-1 is fetched while row i is processed.  xs must be
+// 16-byte aligned.  On return xs holds x and xr is filled.
 template <int MAXM>
 __device__ __forceinline__ void wave_backsub (const double *U, size_t ldu, const double *y, size_t ys, int n, int lane,
-                                              double (&xr)[MAXM], double *xs)
+                                              double (&xr)[MAXM], double *xs, const int *rmap = nullptr)
 {
 #pragma unroll
   for (int m = 0; m < MAXM; m++)
     xr[m] = 0.0;
+  int ri = rmap ? __builtin_amdgcn_readfirstlane (rmap[n - 1]) : n - 1;
+  double cu[MAXM], cy, cd;
+#pragma unroll
+  for (int m = 0; m < MAXM; m++)
+  {
+    const int j = lane + 64 * m;
+    cu[m] = j < n ? U[(size_t) ri * ldu + j] : 0.0;
+  }
+  cy = y[(size_t) ri * ys];
+  cd = U[(size_t) ri * ldu + (n - 1)];
   for (int i = n - 1; i >= 0; i--)
   {
-    const double *Ui = U + (size_t) i * ldu;
-    double p[MAXM];
+    double nu[MAXM], ny = 0.0, nd = 1.0;
+    if (i > 0)
+    {
+      const int rn = rmap ? __builtin_amdgcn_readfirstlane (rmap[i - 1]) : i - 1;
+#pragma unroll
+      for (int m = 0; m < MAXM; m++)
+      {
+        const int j = lane + 64 * m;
+        nu[m] = j < n ? U[(size_t) rn * ldu + j] : 0.0;
+      }
+      ny = y[(size_t) rn * ys];
+      nd = U[(size_t) rn * ldu + (i - 1)];
+    }
 #pragma unroll
     for (int m = 0; m < MAXM; m++)
     {
       const int j = lane + 64 * m;
-      p[m] = (j > i && j < n) ? Ui[j] * xr[m] : 0.0;
+      if (j > i && j < n)
+        xs[j] = cu[m] * xr[m];
     }
-    double tmp = y[(size_t) i * ys];
-#pragma unroll
-    for (int m = 0; m < MAXM; m++)
+    // xs is 16-byte aligned: one odd leading term, then aligned pairs, in ascending j
+    double tmp = cy;
+    int j = i + 1;
+    if ((j & 1) && j < n)
     {
-      const int j0 = max (i + 1, 64 * m), j1 = min (n, 64 * m + 64);
-      for (int j = j0; j < j1; j++)
-        tmp = tmp - rl_f64 (p[m], j - 64 * m);
+      tmp = tmp - xs[j];
+      j++;
     }
-    const double xi = tmp / Ui[i];
+    for (; j + 4 <= n; j += 4)
+    {
+      const double2 p01 = *reinterpret_cast<const double2 *> (xs + j);
+      const double2 p23 = *reinterpret_cast<const double2 *> (xs + j + 2);
+      tmp = tmp - p01.x;
+      tmp = tmp - p01.y;
+      tmp = tmp - p23.x;
+      tmp = tmp - p23.y;
+    }
+    for (; j < n; j++)
+      tmp = tmp - xs[j];
+    const double xi = tmp / cd;
 #pragma unroll
     for (int m = 0; m < MAXM; m++)
       if (lane + 64 * m == i)
         xr[m] = xi;
-  }
-  if (xs)
-  {
 #pragma unroll
     for (int m = 0; m < MAXM; m++)
-      if (lane + 64 * m < n)
-        xs[lane + 64 * m] = xr[m];
+      cu[m] = nu[m];
+    cy = ny;
+    cd = nd;
+  }
+#pragma unroll
+  for (int m = 0; m < MAXM; m++)
+  {
+    const int j = lane + 64 * m;
+    if (j < n)
+      xs[j] = xr[m];
   }
 }
 
 // ============================================================================================
 // k_wave_f64: one wave per matrix
 // ============================================================================================
+// Per wave: the lane whose row is chosen as pivot at step k keeps that row's remaining columns
+// (U(k, k..n-1)) and y_k in its registers from then on; the other lanes read the row from a small
+// LDS broadcast buffer.  A is loaded coalesced through a staging tile of SR rows; LDS per wave is
+// the tile, the broadcast row and x (plus the multipliers when the factors are kept).
+template <int NC>
+struct WaveF64Layout
+{
+  static constexpr int SR = NC < 16 ? NC : 16;   // rows per staging pass of the A load
+  static constexpr int SP = NC | 1;              // staging pitch (odd: fewer bank conflicts)
+  static constexpr int BR = NC + 2;              // broadcast row: [t] = U(k, k+t), [NC] = y_k
+  static constexpr int LDL = NC + 1;             // KEEP: multipliers of one row
+  static constexpr int base (bool keep)
+  {
+    return ((SR * SP + 1) & ~1) + BR + NC + (keep ? NC * LDL : 0);
+  }
+};
+
 template <int NC, int WPB, bool KEEP>
-__global__ __launch_bounds__ (64 * WPB) void k_wave_f64 (SolveArgs s)
+__global__ __launch_bounds__ (64 * WPB, NC == 64 ? 3 : 4) void k_wave_f64 (SolveArgs s)
 {
   extern __shared__ double smem[];
-  constexpr int LDU = NC + 2;                        // U row k: [t] = U(k, k+t), [NC] = y_k
-  constexpr int LDL = NC + 1;
-  constexpr int PER_WAVE = NC * LDU + (KEEP ? NC * LDL : 0);
+  using Lay = WaveF64Layout<NC>;
+  constexpr int SR = Lay::SR, SP = Lay::SP, LDL = Lay::LDL;
+  constexpr int PER_WAVE = (Lay::base (KEEP) + 1) & ~1;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int neff = list_len (s);
-  double *U = smem + wv * PER_WAVE;
-  double *Ls = U + NC * LDU;                          // KEEP: Ls[row * LDL + k] = multiplier
+  double *stg = smem + wv * PER_WAVE;
+  double *brow = stg + ((SR * SP + 1) & ~1);
+  double *xs = brow + Lay::BR;
+  double *Ls = xs + NC;                               // KEEP: Ls[row * LDL + k] = multiplier
   for (int item = blockIdx.x * WPB + wv; item < neff; item += gridDim.x * WPB)
   {
     wave_lds_fence ();
@@ -106,19 +166,29 @@ __global__ __launch_bounds__ (64 * WPB) void k_wave_f64 (SolveArgs s)
       continue;
     }
 
-    // coalesced load of A through LDS (odd pitch: at most 2-way bank conflicts on the row reads)
-    const int pitch = n | 1;
-    for (int r = 0; r < n; r++)
-      for (int c = lane; c < n; c += 64)
-        U[r * pitch + c] = A[(size_t) r * n + c];
-    wave_lds_fence ();
+    // coalesced load of A, SR rows at a time, through the staging tile
     const bool in = lane < n;
     double a[NC];
 #pragma unroll
     for (int j = 0; j < NC; j++)
-      a[j] = (in && j < n) ? U[lane * pitch + j] : 0.0;
+      a[j] = 0.0;
+    for (int r0 = 0; r0 < n; r0 += SR)
+    {
+      const int nr = min (SR, n - r0);
+      for (int r = 0; r < nr; r++)
+        for (int c = lane; c < n; c += 64)
+          stg[r * SP + c] = A[(size_t) (r0 + r) * n + c];
+      wave_lds_fence ();
+      if (lane >= r0 && lane < r0 + nr)
+      {
+#pragma unroll
+        for (int j = 0; j < NC; j++)
+          if (j < n)
+            a[j] = stg[(lane - r0) * SP + j];
+      }
+      wave_lds_fence ();
+    }
     double bb = in ? b[lane] : 0.0;
-    wave_lds_fence ();
 
     bool alive = in;
     int pos = lane;
@@ -127,14 +197,13 @@ __global__ __launch_bounds__ (64 * WPB) void k_wave_f64 (SolveArgs s)
     {
       const int q = wave_pivot_lane (alive, pivot_key (a[0]), pos);
       const int ppos = __builtin_amdgcn_readlane (pos, q);
-      double *Uk = U + k * LDU;
       if (lane == q)
       {
 #pragma unroll
         for (int j = 0; j < NC; j += 2)
           if (j < n - k)
-            *reinterpret_cast<double2 *> (Uk + j) = make_double2 (a[j], a[j + 1]);
-        Uk[NC] = bb;
+            *reinterpret_cast<double2 *> (brow + j) = make_double2 (a[j], a[j + 1]);
+        brow[NC] = bb;
       }
       if (KEEP && lane == 0 && s.piv)
         s.piv[s.offV[cell] + k] = ppos;
@@ -142,22 +211,22 @@ __global__ __launch_bounds__ (64 * WPB) void k_wave_f64 (SolveArgs s)
         pos = ppos;
       if (lane == q)
       {
-        alive = false;
+        alive = false;        // from here on this lane holds U(k, k..) and y_k
         pos = k;
       }
       wave_lds_fence ();
-      const double piv = Uk[0];
+      const double piv = brow[0];
       singular |= (piv == 0.0);
       if (alive)
       {
         const double l = gsl_multiplier (a[0], piv);
         if (KEEP)
           Ls[lane * LDL + k] = l;
-        bb = bb - l * Uk[NC];
+        bb = bb - l * brow[NC];
 #pragma unroll
         for (int j = 1; j < NC; j++)
           if (j < n - k)
-            a[j - 1] = a[j] - l * Uk[j];
+            a[j - 1] = a[j] - l * brow[j];
       }
     }
     wave_lds_fence ();
@@ -168,9 +237,10 @@ __global__ __launch_bounds__ (64 * WPB) void k_wave_f64 (SolveArgs s)
       double *out = s.LU + s.offA[cell] + (size_t) pos * n;
       for (int j = 0; j < pos; j++)
         out[j] = Ls[lane * LDL + j];
-      const double *Ur = U + pos * LDU;
-      for (int j = pos; j < n; j++)
-        out[j] = Ur[j - pos];
+#pragma unroll
+      for (int t = 0; t < NC; t++)
+        if (t < n - pos)
+          out[pos + t] = a[t];
     }
     if (singular)
     {
@@ -179,23 +249,25 @@ __global__ __launch_bounds__ (64 * WPB) void k_wave_f64 (SolveArgs s)
       continue;
     }
 
-    // back substitution: U row i is U + i*LDU shifted by i, so hand wave_backsub a base that
-    // makes row i's column index absolute
-    double xr[1];
+    // back substitution in dtrsv order: the lane holding U row i forms its chain from x in LDS
+    for (int i = n - 1; i >= 0; i--)
     {
-      xr[0] = 0.0;
-      for (int i = n - 1; i >= 0; i--)
+      if (in && pos == i)
       {
-        const double *Ui = U + i * LDU;   // Ui[t] = U(i, i+t)
-        const double p = (lane > i && lane < n) ? Ui[lane - i] * xr[0] : 0.0;
-        double tmp = Ui[NC];
-        for (int j = i + 1; j < n; j++)
-          tmp = tmp - rl_f64 (p, j);
-        const double xi = tmp / Ui[0];
-        if (lane == i)
-          xr[0] = xi;
+        double tmp = bb;
+#pragma unroll
+        for (int t = 1; t < NC; t++)
+        {
+          if (t < n - i)
+          {
+            const double pr = a[t] * xs[i + t];
+            tmp = tmp - pr;
+          }
+        }
+        xs[i] = tmp / a[0];
       }
     }
+    wave_lds_fence ();
 
     // residual check t = A x (gsl_blas_dgemv order) and codes 2 / 3 of the first failing row
     int fail = 0;
@@ -206,8 +278,8 @@ __global__ __launch_bounds__ (64 * WPB) void k_wave_f64 (SolveArgs s)
 #pragma unroll 8
       for (int j = 0; j < n; j++)
       {
-        const double p = rl_f64 (xr[0], j) * Ai[j];
-        t = t + p;
+        const double pr = Ai[j] * xs[j];
+        t = t + pr;
       }
       const double tv = 0.0 + t;
       fail = residual_row_code (tv, b[lane]);
@@ -217,7 +289,7 @@ __global__ __launch_bounds__ (64 * WPB) void k_wave_f64 (SolveArgs s)
     if (fb)
       code = __builtin_amdgcn_readlane (fail, __ffsll ((long long) fb) - 1);
     if (in)
-      s.X[s.offV[cell] + lane] = xr[0];
+      s.X[s.offV[cell] + lane] = xs[lane];
     if (lane == 0)
       s.codes[cell] = code;
   }
@@ -226,14 +298,24 @@ __global__ __launch_bounds__ (64 * WPB) void k_wave_f64 (SolveArgs s)
 // ============================================================================================
 // k_lds_f64: one workgroup per matrix, [A | b] resident in LDS
 // ============================================================================================
+//
+// Rows stay where they were loaded; phys[q] names the row at position q (GSL's row order after
+// its swaps) and prow[k] the pivot row of step k, i.e. the row that ends at position k.  Step k: every thread knows the pivot (row p at position pq) from the previous
+// step's candidates; the pivot row's columns k+1..n go to registers, and the waves eliminate
+// positions k+1..n-1 (the row displaced from position k takes position pq), RU rows per wave
+// at a time.  Lane 0 of each wave also holds column k+1 of every row it updates, so it keeps the
+// wave's candidate for the next pivot (largest |a|, ties to the smallest position) and one
+// barrier per step suffices; the candidate slots alternate between two buffers by step parity.
 template <int BS, bool KEEP>
 __global__ __launch_bounds__ (BS) void k_lds_f64 (SolveArgs s)
 {
   extern __shared__ double M[];
-  __shared__ long long r_key[BS / 64];
-  __shared__ int r_pos[BS / 64];
-  __shared__ int s_fail;
   constexpr int NW = BS / 64;
+  constexpr int RU = 4;            // rows per wave per pass
+  constexpr int TC = 3;            // column chunks of 64 per row (n + 1 <= 192)
+  __shared__ long long r_key[2][NW];
+  __shared__ int r_pos[2][NW], r_row[2][NW];
+  __shared__ int s_fail;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int neff = list_len (s);
   for (int item = blockIdx.x; item < neff; item += gridDim.x)
@@ -241,7 +323,9 @@ __global__ __launch_bounds__ (BS) void k_lds_f64 (SolveArgs s)
     const int cell = s.list[item];
     const int n = s.n[cell];
     const int ld = (n + 1) | 1;
-    double *xs = M + (size_t) n * ld;
+    double *xs = M + (((size_t) n * ld + 1) & ~(size_t) 1);   // 16-byte aligned
+    int *phys = (int *) (xs + n);
+    int *prow = phys + n;
     const double *A = s.A + s.offA[cell];
     const double *b = s.B + s.offV[cell];
 
@@ -249,64 +333,121 @@ __global__ __launch_bounds__ (BS) void k_lds_f64 (SolveArgs s)
       for (int c = lane; c < n; c += 64)
         M[r * ld + c] = A[(size_t) r * n + c];
     for (int r = tid; r < n; r += BS)
+    {
       M[r * ld + n] = b[r];
+      phys[r] = r;
+    }
     if (tid == 0)
       s_fail = 0x7fffffff;
+    __syncthreads ();
+
+    // candidates for the first pivot: one row per thread (n <= BS)
+    {
+      const long long key = tid < n ? pivot_key (M[tid * ld]) : -1;
+      const int wl = wave_pivot_lane (tid < n, key, tid);
+      if (lane == 0)
+      {
+        r_key[0][wv] = wl >= 0 ? (long long) rl_u64 ((unsigned long long) key, wl) : -1;
+        r_pos[0][wv] = wl >= 0 ? __builtin_amdgcn_readlane (tid, wl) : 0x7fffffff;
+        r_row[0][wv] = r_pos[0][wv];
+      }
+    }
     __syncthreads ();
 
     int singular = 0;
     for (int k = 0; k < n; k++)
     {
-      // pivot: rows k.. (one per thread for n <= BS), first strict max by row position
-      long long key = -1;
-      int pos = 0x7fffffff;
-      for (int i = k + tid; i < n; i += BS)
-      {
-        const long long q = pivot_key (M[i * ld + k]);
-        if (q > key)
-        {
-          key = q;
-          pos = i;
-        }
-      }
-      const int wl = wave_pivot_lane (key >= 0, key, pos);
-      if (lane == 0)
-      {
-        r_key[wv] = wl >= 0 ? (long long) rl_u64 ((unsigned long long) key, wl) : -1;
-        r_pos[wv] = wl >= 0 ? __builtin_amdgcn_readlane (pos, wl) : 0x7fffffff;
-      }
-      __syncthreads ();
-      long long bk = r_key[0];
-      int p = r_pos[0];
-#pragma unroll
-      for (int w = 1; w < NW; w++)
-        if (r_key[w] > bk || (r_key[w] == bk && r_pos[w] < p))
-        {
-          bk = r_key[w];
-          p = r_pos[w];
-        }
-      if (KEEP && tid == 0 && s.piv)
-        s.piv[s.offV[cell] + k] = p;
-      if (p != k)
-        for (int c = tid; c <= n; c += BS)
-        {
-          const double t = M[k * ld + c];
-          M[k * ld + c] = M[p * ld + c];
-          M[p * ld + c] = t;
-        }
-      __syncthreads ();
-      const double piv = M[k * ld + k];
+      const int cb = k & 1, nb = cb ^ 1;
+      int pq, p;
+      combine_pivot_candidates (r_key[cb], r_pos[cb], r_row[cb], NW, lane, pq, p);
+      const int q0 = __builtin_amdgcn_readfirstlane (phys[k]);
+      const double *Mp = M + p * ld;
+      const double piv = Mp[k];
       singular |= (piv == 0.0);
       const bool recip = fabs (piv) >= DBL_MIN;
       const double sc = 1.0 / piv;
-      for (int i = k + 1 + wv; i < n; i += NW)
+      double ur[TC];
+#pragma unroll
+      for (int t = 0; t < TC; t++)
       {
-        const double aik = M[i * ld + k];
-        const double l = recip ? aik * sc : aik / piv;
-        for (int j = k + 1 + lane; j <= n; j += 64)
-          M[i * ld + j] = M[i * ld + j] - l * M[k * ld + j];
-        if (lane == 0)
-          M[i * ld + k] = l;
+        const int j = k + 1 + lane + 64 * t;
+        ur[t] = j <= n ? Mp[j] : 0.0;
+      }
+      if (KEEP && tid == 0 && s.piv)
+        s.piv[s.offV[cell] + k] = pq;
+
+      long long ck = -1;
+      int cp = 0x7fffffff, cr = 0;
+      for (int qb = k + 1 + wv; qb < n; qb += RU * NW)
+      {
+        int ri[RU];
+        double ak[RU], v[RU][TC];
+#pragma unroll
+        for (int u = 0; u < RU; u++)
+        {
+          const int q = qb + u * NW;
+          ri[u] = q < n ? ((q == pq) ? q0 : __builtin_amdgcn_readfirstlane (phys[q])) : p;
+        }
+#pragma unroll
+        for (int u = 0; u < RU; u++)
+        {
+          const double *Mi = M + ri[u] * ld;
+          ak[u] = Mi[k];
+#pragma unroll
+          for (int t = 0; t < TC; t++)
+          {
+            const int j = k + 1 + lane + 64 * t;
+            v[u][t] = j <= n ? Mi[j] : 0.0;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < RU; u++)
+        {
+          if (qb + u * NW < n)
+          {
+            double *Mi = M + ri[u] * ld;
+            const double l = recip ? ak[u] * sc : ak[u] / piv;
+#pragma unroll
+            for (int t = 0; t < TC; t++)
+            {
+              const int j = k + 1 + lane + 64 * t;
+              if (j <= n)
+                Mi[j] = v[u][t] - l * ur[t];
+            }
+            if (lane == 0)
+              Mi[k] = l;
+          }
+        }
+        // lane 0 holds column k+1 (t = 0) of each row: the next pivot's candidates
+#pragma unroll
+        for (int u = 0; u < RU; u++)
+        {
+          const int q = qb + u * NW;
+          if (q < n && k + 1 < n)
+          {
+            const double l = recip ? ak[u] * sc : ak[u] / piv;
+            const long long key = pivot_key (v[u][0] - l * ur[0]);
+            if (key > ck || (key == ck && q < cp))
+            {
+              ck = key;
+              cp = q;
+              cr = ri[u];
+            }
+          }
+        }
+      }
+      if (lane == 0)
+      {
+        r_key[nb][wv] = ck;
+        r_pos[nb][wv] = cp;
+        r_row[nb][wv] = cr;
+      }
+      if (tid == 0)
+      {
+        // GSL swaps positions k and pq.  phys[k] is not rewritten (it was read at the top of
+        // this step, possibly by a later wave); positions > k other than pq are read above.
+        phys[pq] = q0;
+        prow[k] = p;
       }
       __syncthreads ();
     }
@@ -315,8 +456,11 @@ __global__ __launch_bounds__ (BS) void k_lds_f64 (SolveArgs s)
     {
       double *out = s.LU + s.offA[cell];
       for (int r = wv; r < n; r += NW)
+      {
+        const double *Mr = M + prow[r] * ld;
         for (int c = lane; c < n; c += 64)
-          out[(size_t) r * n + c] = M[r * ld + c];
+          out[(size_t) r * n + c] = Mr[c];
+      }
     }
     if (singular)
     {
@@ -329,7 +473,7 @@ __global__ __launch_bounds__ (BS) void k_lds_f64 (SolveArgs s)
     if (wv == 0)
     {
       double xr[3];
-      wave_backsub<3> (M, (size_t) ld, M + n, (size_t) ld, n, lane, xr, xs);
+      wave_backsub<3> (M, (size_t) ld, M + n, (size_t) ld, n, lane, xr, xs, prow);
     }
     __syncthreads ();
 
@@ -355,53 +499,47 @@ __global__ __launch_bounds__ (BS) void k_lds_f64 (SolveArgs s)
   }
 }
 
-// U12 (trsm) and A22 (gemm) work of one column c >= k0 + nbk of the blocked LU, after the panel's
-// row swaps.  Kept out of line so the L11 panel values are read per column instead of being
-// hoisted into registers across the column loop.
-template <int NB>
-__device__ __forceinline__ void blocked_column (double *W, const double *P, int n, int k0, int nbk, int c)
+// A value every lane of the wave holds (an LDS broadcast read), moved to a scalar register.
+__device__ __forceinline__ int uni (int v)
 {
-  constexpr int PP = NB + 1;
-  const int k1 = k0 + nbk;
-  double u[NB];
-#pragma unroll
-  for (int kk = 0; kk < NB; kk++)
-    u[kk] = (kk < nbk) ? W[(size_t) (k0 + kk) * n + c] : 0.0;
-  // U12 = L11^-1 A12, row by row: u[r] receives its updates in ascending kk, the order of the
-  // right-looking elimination (and of GSL's dtrsm, cblas/source_trsm_r.h)
-#pragma unroll
-  for (int r = 1; r < NB; r++)
-#pragma unroll
-    for (int kk = 0; kk < r; kk++)
-      u[r] = u[r] - P[r * PP + kk] * u[kk];
-#pragma unroll
-  for (int kk = 0; kk < NB; kk++)
-    if (kk < nbk)
-      W[(size_t) (k0 + kk) * n + c] = u[kk];
-  for (int i = k1; i < n; i++)
-  {
-    const double *Li = P + (i - k0) * PP;
-    double a = W[(size_t) i * n + c];
-#pragma unroll
-    for (int kk = 0; kk < NB; kk++)
-      a = a - Li[kk] * u[kk];
-    W[(size_t) i * n + c] = a;
-  }
+  return __builtin_amdgcn_readfirstlane (v);
+}
+
+// Row `row` of an n x n row-major matrix (row wave-uniform: a scalar pointer).
+__device__ __forceinline__ double *wrow (double *W, int n, int row)
+{
+  return W + (size_t) (unsigned) row * (unsigned) n;
 }
 
 // ============================================================================================
-// k_blocked_f64: right-looking blocked LU with an NB-column panel in LDS
+// k_blocked_f64: right-looking blocked LU, NB-column panel in LDS, trailing matrix in a
+// per-workgroup HBM workspace (MALL / L2 resident between panels)
 // ============================================================================================
-template <int BS, int NB, int MAXM, bool KEEP>
+//
+// Per panel k0 .. k1 = k0 + NB:
+//   1. the panel (all rows from k0 down, NB columns and b) is factored in LDS.  Rows are not
+//      moved: pos[r] / phys[q] track which panel row sits at which position, so one column costs
+//      two barriers (pivot candidates, then elimination against the pivot row, which nobody
+//      writes in that step).  GSL's pivot rule and tie break use the positions.
+//   2. the panel is written back in position order; the net row permutation of the panel (at
+//      most 2 NB moved rows) is applied to every other column as one gather / scatter per column,
+//      and U12 = L11^-1 A12 is formed per column (ascending kk, dtrsm's order) into W and LDS.
+//   3. A22 -= L21 U12 in tiles of RB rows x 64 columns per wave; each element takes its NB
+//      updates in ascending kk with the product rounded first, the order of GSL's recursion.
+// Then the back substitution runs from LDS-staged chunks of U and the workgroup forms the
+// residual check.
+template <int BS, int NB, bool US, int MAXM, bool KEEP>
 __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s, double *work, int64_t wstride)
 {
-  extern __shared__ double P[];                  // panel rows x (NB+1); column NB holds b
-  constexpr int PP = NB + 1;
+  extern __shared__ double smem[];
+  constexpr int PP = NB + 2;           // panel row pitch (even: 16-byte aligned rows); column NB = b
   constexpr int NW = BS / 64;
+  constexpr int RB = 8;                // rows per trailing-update tile
   __shared__ long long r_key[NW];
-  __shared__ int r_pos[NW];
+  __shared__ int r_pos[NW], r_row[NW];
   __shared__ int s_ipiv[NB];
-  __shared__ int s_fail;
+  __shared__ int mv_dst[NB], mv_src[NB];
+  __shared__ int s_nmv, s_fail;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 
   const int neff = list_len (s);
@@ -413,10 +551,29 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s, double *wo
     const double *b = s.B + s.offV[cell];
     double *W = (KEEP && s.LU) ? (s.LU + s.offA[cell]) : (work + (size_t) blockIdx.x * wstride);
     double *wb = (KEEP && s.LU) ? (work + (size_t) blockIdx.x * wstride) : (W + (size_t) n * n);
-    double *xs = P + (size_t) n * PP;
+    double *P = smem;
+    double *Us = P + (size_t) n * PP;
+    double *xs = Us + (US ? (size_t) NB * n : 0);
+    int *pos = (int *) (xs + n);       // position of panel row r (both relative to k0)
+    int *phys = pos + n;               // panel row at position q
 
-    for (int64_t e = tid; e < (int64_t) n * n; e += BS)
-      W[e] = A[e];
+    {
+      // eight loads in flight per thread, then their stores
+      const int64_t tot = (int64_t) n * n;
+      int64_t e = tid;
+      for (; e + 7 * BS < tot; e += 8 * BS)
+      {
+        double t[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+          t[u] = A[e + u * BS];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+          W[e + u * BS] = t[u];
+      }
+      for (; e < tot; e += BS)
+        W[e] = A[e];
+    }
     for (int i = tid; i < n; i += BS)
       wb[i] = b[i];
     if (tid == 0)
@@ -428,104 +585,230 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s, double *wo
     {
       const int nbk = min (NB, n - k0);
       const int rows = n - k0;
+      const int k1 = k0 + nbk;
       for (int r = tid; r < rows; r += BS)
       {
         const double *src = W + (size_t) (k0 + r) * n + k0;
-        double *dst = P + r * PP;
+        double *dst = P + (size_t) r * PP;
         for (int c = 0; c < nbk; c++)
           dst[c] = src[c];
         dst[NB] = wb[k0 + r];
+        pos[r] = r;
+        phys[r] = r;
       }
+      if (tid == 0)
+        s_nmv = 0;
       __syncthreads ();
 
-      // ---- panel factorization ----
+      // ---- 1. panel factorization ----
       for (int kk = 0; kk < nbk; kk++)
       {
         long long key = -1;
-        int pos = 0x7fffffff;
-        for (int r = kk + tid; r < rows; r += BS)
+        int bp = 0x7fffffff, br = 0;
+        for (int r = tid; r < rows; r += BS)
         {
-          const long long q = pivot_key (P[r * PP + kk]);
-          if (q > key)
+          const int q = pos[r];
+          if (q >= kk)
           {
-            key = q;
-            pos = r;
+            const long long k = pivot_key (P[(size_t) r * PP + kk]);
+            if (k > key || (k == key && q < bp))
+            {
+              key = k;
+              bp = q;
+              br = r;
+            }
           }
         }
-        const int wl = wave_pivot_lane (key >= 0, key, pos);
+        const int wl = wave_pivot_lane (key >= 0, key, bp);
         if (lane == 0)
         {
           r_key[wv] = wl >= 0 ? (long long) rl_u64 ((unsigned long long) key, wl) : -1;
-          r_pos[wv] = wl >= 0 ? __builtin_amdgcn_readlane (pos, wl) : 0x7fffffff;
+          r_pos[wv] = wl >= 0 ? __builtin_amdgcn_readlane (bp, wl) : 0x7fffffff;
+          r_row[wv] = wl >= 0 ? __builtin_amdgcn_readlane (br, wl) : 0;
         }
         __syncthreads ();
-        long long bk = r_key[0];
-        int p = r_pos[0];
-#pragma unroll
-        for (int w = 1; w < NW; w++)
-          if (r_key[w] > bk || (r_key[w] == bk && r_pos[w] < p))
-          {
-            bk = r_key[w];
-            p = r_pos[w];
-          }
-        if (tid == 0)
-          s_ipiv[kk] = p;
-        if (p != kk && tid < PP)
-        {
-          const double t = P[kk * PP + tid];
-          P[kk * PP + tid] = P[p * PP + tid];
-          P[p * PP + tid] = t;
-        }
-        __syncthreads ();
-        const double piv = P[kk * PP + kk];
+        int pq, p;
+        combine_pivot_candidates (r_key, r_pos, r_row, NW, lane, pq, p);
+        const double *Pp = P + (size_t) p * PP;
+        const double piv = Pp[kk];
         singular |= (piv == 0.0);
         const bool recip = fabs (piv) >= DBL_MIN;
         const double sc = 1.0 / piv;
-        for (int r = kk + 1 + tid; r < rows; r += BS)
+        // every row still below the pivot: the row at position kk (moving to pq) included
+        // (the whole row and the pivot row are read into registers, updated, written back)
+        for (int r = tid; r < rows; r += BS)
         {
-          double *Pr = P + r * PP;
-          const double l = recip ? Pr[kk] * sc : Pr[kk] / piv;
-          Pr[kk] = l;
-          for (int c = kk + 1; c < nbk; c++)
-            Pr[c] = Pr[c] - l * P[kk * PP + c];
-          Pr[NB] = Pr[NB] - l * P[kk * PP + NB];
+          if (r == p || pos[r] < kk)
+            continue;
+          double *Pr = P + (size_t) r * PP;
+          const double ark = Pr[kk];
+          double v[PP];
+#pragma unroll
+          for (int c = 0; c < PP; c += 2)
+          {
+            const double2 x = *reinterpret_cast<const double2 *> (Pr + c);
+            v[c] = x.x;
+            v[c + 1] = x.y;
+          }
+          double l;
+          if (recip)
+            l = ark * sc;
+          else
+            l = ark / piv;
+#pragma unroll
+          for (int c = 0; c < PP; c++)
+          {
+            if (c == NB || (c > kk && c < nbk))
+              v[c] = v[c] - l * Pp[c];
+            else if (c == kk)
+              v[c] = l;
+          }
+#pragma unroll
+          for (int c = 0; c < PP; c += 2)
+            *reinterpret_cast<double2 *> (Pr + c) = make_double2 (v[c], v[c + 1]);
+        }
+        if (tid == 0)
+        {
+          // GSL swaps the rows at positions kk and pq.  The elimination above reads pos[] of
+          // other rows only; the row moved here is eliminated whether it reads kk or pq.
+          const int q0 = phys[kk];
+          phys[pq] = q0;
+          pos[q0] = pq;
+          phys[kk] = p;
+          pos[p] = kk;
+          s_ipiv[kk] = pq;
         }
         __syncthreads ();
       }
 
-      // ---- write the panel back, pivots out ----
-      for (int r = tid; r < rows; r += BS)
+      // ---- 2a. panel back in position order; rows moved below the panel ----
+      for (int q = tid; q < rows; q += BS)
       {
-        double *dst = W + (size_t) (k0 + r) * n + k0;
-        const double *src = P + r * PP;
+        const int r = phys[q];
+        const double *src = P + (size_t) r * PP;
+        double *dst = W + (size_t) (k0 + q) * n + k0;
         for (int c = 0; c < nbk; c++)
           dst[c] = src[c];
-        wb[k0 + r] = src[NB];
+        wb[k0 + q] = src[NB];
+        if (q >= nbk && r != q)
+        {
+          const int m = atomicAdd (&s_nmv, 1);
+          mv_dst[m] = q;
+          mv_src[m] = r;
+        }
       }
       if (KEEP && s.piv && tid < nbk)
         s.piv[s.offV[cell] + k0 + tid] = k0 + s_ipiv[tid];
+      __syncthreads ();
 
-      // ---- per column outside the panel: row swaps, then U12 (trsm) and A22 (gemm) ----
-      const int k1 = k0 + nbk;
+      // ---- 2b. other columns: row permutation, then U12 ----
+      // Row indices are wave-uniform (LDS broadcasts moved to scalar registers): each access is
+      // a scalar row base plus the lane's unsigned column offset.
+      const int nmv = uni (s_nmv);
+      const int last = nbk - 1;
       for (int c = tid; c < n; c += BS)
       {
         if (c >= k0 && c < k1)
           continue;
-        for (int kk = 0; kk < nbk; kk++)
+        const unsigned cu = (unsigned) c;
+        double u[NB];
+        // all loads first (indices clamped so every one is in bounds, conditions wave-uniform)
+#pragma unroll
+        for (int kk = 0; kk < NB; kk++)
+          u[kk] = wrow (W, n, k0 + uni (phys[min (kk, last)]))[cu];
+        if (US)
         {
-          const int p = s_ipiv[kk];
-          if (p != kk)
+          // park the permuted rows in this column's U12 slots (free until written below) so the
+          // displaced rows' registers are not live at the same time
+#pragma unroll
+          for (int kk = 0; kk < NB; kk++)
+            Us[kk * n + c] = u[kk];
+          __asm__ volatile ("" ::: "memory");
+        }
+        if (nmv > 0)
+        {
+          double v[NB];
+#pragma unroll
+          for (int m = 0; m < NB; m++)
+            v[m] = wrow (W, n, k0 + uni (mv_src[min (m, nmv - 1)]))[cu];
+#pragma unroll
+          for (int m = 0; m < NB; m++)
+            if (m < nmv)
+              wrow (W, n, k0 + uni (mv_dst[m]))[cu] = v[m];
+        }
+        if (US)
+        {
+          __asm__ volatile ("" ::: "memory");
+#pragma unroll
+          for (int kk = 0; kk < NB; kk++)
+            u[kk] = Us[kk * n + c];
+        }
+        if (c >= k1)
+        {
+          // a column right of the panel exists only when the panel is full (nbk == NB)
+#pragma unroll
+          for (int r = 1; r < NB; r++)
           {
-            double *x0 = W + (size_t) (k0 + kk) * n + c;
-            double *x1 = W + (size_t) (k0 + p) * n + c;
-            const double t = *x0;
-            *x0 = *x1;
-            *x1 = t;
+            const double *Lr = P + (size_t) uni (phys[r]) * PP;
+#pragma unroll
+            for (int kk = 0; kk < r; kk++)
+              u[r] = u[r] - Lr[kk] * u[kk];
+          }
+#pragma unroll
+          for (int kk = 0; kk < NB; kk++)
+          {
+            wrow (W, n, k0 + kk)[cu] = u[kk];
+            if (US)
+              Us[kk * n + c] = u[kk];
           }
         }
-        if (c < k0)
-          continue;
-        blocked_column<NB> (W, P, n, k0, nbk, c);
+        else
+        {
+#pragma unroll
+          for (int kk = 0; kk < NB; kk++)
+            if (kk < nbk && uni (phys[min (kk, last)]) != kk)
+              wrow (W, n, k0 + kk)[cu] = u[kk];
+        }
+      }
+      __syncthreads ();
+
+      // ---- 3. A22 -= L21 U12 (only when the panel is full) ----
+      const int nr = n - k1;
+      if (nr > 0)
+      {
+        const int nch = (nr + 63) >> 6;
+        const int nit = nch * ((nr + RB - 1) / RB);
+        for (int it = wv; it < nit; it += NW)
+        {
+          const int ch = it % nch;
+          const int i0 = k1 + (it / nch) * RB;
+          const int c = k1 + ch * 64 + lane;
+          const unsigned cu = (unsigned) c;
+          const bool cv = c < n;
+          double a[RB];
+#pragma unroll
+          for (int r = 0; r < RB; r++)
+            a[r] = (cv && i0 + r < n) ? wrow (W, n, i0 + r)[cu] : 0.0;
+          double u[NB];
+#pragma unroll
+          for (int kk = 0; kk < NB; kk++)
+            u[kk] = cv ? (US ? Us[kk * n + c] : wrow (W, n, k0 + kk)[cu]) : 0.0;
+#pragma unroll
+          for (int r = 0; r < RB; r++)
+          {
+            if (i0 + r < n)
+            {
+              const double *Li = P + (size_t) uni (phys[i0 + r - k0]) * PP;
+#pragma unroll
+              for (int kk = 0; kk < NB; kk++)
+                a[r] = a[r] - Li[kk] * u[kk];
+            }
+          }
+#pragma unroll
+          for (int r = 0; r < RB; r++)
+            if (cv && i0 + r < n)
+              wrow (W, n, i0 + r)[cu] = a[r];
+        }
       }
       __syncthreads ();
     }
@@ -538,12 +821,88 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s, double *wo
       continue;
     }
 
-    if (wv == 0)
+    // ---- back substitution (dtrsv order) from LDS: U rows and y are staged in chunks of CR
+    // rows, double-buffered over the (now free) panel / U12 area; waves 1.. load the next chunk
+    // while wave 0 runs the chain of the current one ----
     {
+      constexpr int CR = US ? 16 : 4;
+      const int pitch = (n + 2) & ~1;                  // U row (n) + y_i, 16-byte aligned rows
+      const int nch = (n + CR - 1) / CR;
+      double *cbuf = smem;                             // 2 x CR x pitch <= n x (PP + NB or 0)
+      auto stage = [&] (int c, int t0, int nt) {
+        const int hi = n - 1 - c * CR;
+        const int lo = max (0, hi - CR + 1);
+        double *dst = cbuf + (size_t) (c & 1) * CR * pitch;
+        const int cnt = (hi - lo + 1) * n;
+        for (int e = t0; e < cnt; e += nt)
+        {
+          const int r = e / n, j = e - r * n;
+          dst[r * pitch + j] = W[(size_t) (lo + r) * n + j];
+        }
+        for (int r = t0; r <= hi - lo; r += nt)
+          dst[r * pitch + n] = wb[lo + r];
+      };
+      stage (0, tid, BS);
+      __syncthreads ();
       double xr[MAXM];
-      wave_backsub<MAXM> (W, (size_t) n, wb, 1, n, lane, xr, xs);
+#pragma unroll
+      for (int m = 0; m < MAXM; m++)
+        xr[m] = 0.0;
+      for (int c = 0; c < nch; c++)
+      {
+        if (wv > 0 && c + 1 < nch)
+          stage (c + 1, tid - 64, BS - 64);
+        if (wv == 0)
+        {
+          const int hi = n - 1 - c * CR;
+          const int lo = max (0, hi - CR + 1);
+          const double *buf = cbuf + (size_t) (c & 1) * CR * pitch;
+          for (int i = hi; i >= lo; i--)
+          {
+            const double *Ur = buf + (i - lo) * pitch;
+#pragma unroll
+            for (int m = 0; m < MAXM; m++)
+            {
+              const int j = lane + 64 * m;
+              if (j > i && j < n)
+                xs[j] = Ur[j] * xr[m];
+            }
+            double tmp = Ur[n];
+            int j = i + 1;
+            if ((j & 1) && j < n)
+            {
+              tmp = tmp - xs[j];
+              j++;
+            }
+            for (; j + 4 <= n; j += 4)
+            {
+              const double2 p01 = *reinterpret_cast<const double2 *> (xs + j);
+              const double2 p23 = *reinterpret_cast<const double2 *> (xs + j + 2);
+              tmp = tmp - p01.x;
+              tmp = tmp - p01.y;
+              tmp = tmp - p23.x;
+              tmp = tmp - p23.y;
+            }
+            for (; j < n; j++)
+              tmp = tmp - xs[j];
+            const double xi = tmp / Ur[i];
+#pragma unroll
+            for (int m = 0; m < MAXM; m++)
+              if (lane + 64 * m == i)
+                xr[m] = xi;
+          }
+        }
+        __syncthreads ();
+      }
+      if (wv == 0)
+      {
+#pragma unroll
+        for (int m = 0; m < MAXM; m++)
+          if (lane + 64 * m < n)
+            xs[lane + 64 * m] = xr[m];
+      }
+      __syncthreads ();
     }
-    __syncthreads ();
 
     for (int i = tid; i < n; i += BS)
     {
@@ -572,13 +931,26 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s, double *wo
 // ============================================================================================
 size_t wave_f64_smem (int nc, int wpb, bool keep)
 {
-  return (size_t) wpb * ((size_t) nc * (nc + 2) + (keep ? (size_t) nc * (nc + 1) : 0)) * sizeof (double);
+  int per = 0;
+  switch (nc)
+  {
+  case 16:
+    per = WaveF64Layout<16>::base (keep);
+    break;
+  case 32:
+    per = WaveF64Layout<32>::base (keep);
+    break;
+  default:
+    per = WaveF64Layout<64>::base (keep);
+    break;
+  }
+  return (size_t) wpb * ((per + 1) & ~1) * sizeof (double);
 }
 
 size_t lds_f64_smem (int n)
 {
   const int ld = (n + 1) | 1;
-  return ((size_t) n * ld + (size_t) n) * sizeof (double);
+  return ((((size_t) n * ld + 1) & ~(size_t) 1) + (size_t) n) * sizeof (double) + 2 * (size_t) n * sizeof (int);
 }
 
 int lds_f64_limit ()
@@ -595,9 +967,21 @@ int lds_f64_limit ()
   return cached;
 }
 
+// Blocked kernel variants: NB = 16 with U12 staged in LDS up to n = 512, NB = 8 reading U12 from
+// the workspace up to n = 1024 (LDS: panel n x (NB+2), U12 NB x n, x, two int maps of n).
+static void blocked_variant (int nmax, int &nb, bool &us)
+{
+  nb = nmax <= 512 ? 16 : 8;
+  us = nmax <= 512;
+}
+
 size_t blocked_f64_smem (int nmax)
 {
-  return ((size_t) nmax * (BLK_NB + 1) + (size_t) nmax) * sizeof (double);
+  int nb;
+  bool us;
+  blocked_variant (nmax, nb, us);
+  return (size_t) nmax * (nb + 2) * sizeof (double) + (us ? (size_t) nb * nmax * sizeof (double) : 0) +
+         (size_t) nmax * sizeof (double) + 2 * (size_t) nmax * sizeof (int);
 }
 
 template <int NC, int WPB>
@@ -636,21 +1020,34 @@ hipError_t launch_lds_f64 (const SolveArgs &a, int nmax, bool keep, int grid, hi
     return hipSuccess;
   const size_t sm = lds_f64_smem (nmax);
   grid = (grid > 0 && grid < a.nlist) ? grid : a.nlist;
-  if (keep)
-    hipLaunchKernelGGL ((k_lds_f64<256, true>), dim3 (grid), dim3 (256), sm, st, a);
+  // two matrices per CU when their LDS allows it (512 threads each), else one with 1024 threads
+  if (2 * (sm + 1024) <= hipla_device_max_smem ())
+  {
+    if (keep)
+      hipLaunchKernelGGL ((k_lds_f64<512, true>), dim3 (grid), dim3 (512), sm, st, a);
+    else
+      hipLaunchKernelGGL ((k_lds_f64<512, false>), dim3 (grid), dim3 (512), sm, st, a);
+  }
   else
-    hipLaunchKernelGGL ((k_lds_f64<256, false>), dim3 (grid), dim3 (256), sm, st, a);
+  {
+    if (keep)
+      hipLaunchKernelGGL ((k_lds_f64<1024, true>), dim3 (grid), dim3 (1024), sm, st, a);
+    else
+      hipLaunchKernelGGL ((k_lds_f64<1024, false>), dim3 (grid), dim3 (1024), sm, st, a);
+  }
   return hipGetLastError ();
 }
 
-template <int MAXM>
+template <int NB, bool US, int MAXM>
 static void launch_blocked_m (const SolveArgs &a, int grid, size_t sm, double *work, int64_t wstride, bool keep,
                               hipStream_t st)
 {
   if (keep)
-    hipLaunchKernelGGL ((k_blocked_f64<256, BLK_NB, MAXM, true>), dim3 (grid), dim3 (256), sm, st, a, work, wstride);
+    hipLaunchKernelGGL ((k_blocked_f64<BLK_BS, NB, US, MAXM, true>), dim3 (grid), dim3 (BLK_BS), sm, st, a, work,
+                        wstride);
   else
-    hipLaunchKernelGGL ((k_blocked_f64<256, BLK_NB, MAXM, false>), dim3 (grid), dim3 (256), sm, st, a, work, wstride);
+    hipLaunchKernelGGL ((k_blocked_f64<BLK_BS, NB, US, MAXM, false>), dim3 (grid), dim3 (BLK_BS), sm, st, a, work,
+                        wstride);
 }
 
 hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, double *work, int64_t wstride, bool keep,
@@ -662,9 +1059,11 @@ hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, double *w
     return hipErrorInvalidValue;
   const size_t sm = blocked_f64_smem (nmax);
   if (nmax <= 256)
-    launch_blocked_m<4> (a, grid, sm, work, wstride, keep, st);
+    launch_blocked_m<16, true, 4> (a, grid, sm, work, wstride, keep, st);
+  else if (nmax <= 512)
+    launch_blocked_m<16, true, 8> (a, grid, sm, work, wstride, keep, st);
   else
-    launch_blocked_m<BLK_MAXM> (a, grid, sm, work, wstride, keep, st);
+    launch_blocked_m<8, false, BLK_MAXM> (a, grid, sm, work, wstride, keep, st);
   return hipGetLastError ();
 }
 
@@ -689,11 +1088,15 @@ hipError_t setup_solve_f64_attrs (size_t max_smem)
   set_smem_attr (k_wave_f64<16, 4, true>, max_smem);
   set_smem_attr (k_wave_f64<32, 4, true>, max_smem);
   set_smem_attr (k_wave_f64<64, 2, true>, max_smem);
-  set_smem_attr (k_lds_f64<256, false>, max_smem);
-  set_smem_attr (k_lds_f64<256, true>, max_smem);
-  set_smem_attr (k_blocked_f64<256, BLK_NB, 4, false>, max_smem);
-  set_smem_attr (k_blocked_f64<256, BLK_NB, 4, true>, max_smem);
-  set_smem_attr (k_blocked_f64<256, BLK_NB, BLK_MAXM, false>, max_smem);
-  set_smem_attr (k_blocked_f64<256, BLK_NB, BLK_MAXM, true>, max_smem);
+  set_smem_attr (k_lds_f64<512, false>, max_smem);
+  set_smem_attr (k_lds_f64<512, true>, max_smem);
+  set_smem_attr (k_lds_f64<1024, false>, max_smem);
+  set_smem_attr (k_lds_f64<1024, true>, max_smem);
+  set_smem_attr (k_blocked_f64<BLK_BS, 16, true, 4, false>, max_smem);
+  set_smem_attr (k_blocked_f64<BLK_BS, 16, true, 4, true>, max_smem);
+  set_smem_attr (k_blocked_f64<BLK_BS, 16, true, 8, false>, max_smem);
+  set_smem_attr (k_blocked_f64<BLK_BS, 16, true, 8, true>, max_smem);
+  set_smem_attr (k_blocked_f64<BLK_BS, 8, false, BLK_MAXM, false>, max_smem);
+  set_smem_attr (k_blocked_f64<BLK_BS, 8, false, BLK_MAXM, true>, max_smem);
   return hipSuccess;
 }

@@ -92,3 +92,33 @@ def test_mixed_singular_goes_to_fp64(hip, oracle_lib):
     X, codes, stats = hip.solve_batch(batch, mode="mixed")
     assert np.all(codes == 4)
     assert np.all(stats["fallback"] == 1)
+
+
+def test_mixed_autotuned_classes_run_concurrently(hip, oracle_lib):
+    """After autotuning, size classes routed straight to the FP64 kernels run concurrently on
+    their own streams (and under graph replay); the blocked ones each use their own workspace
+    slots, so every cell still equals the FP64 oracle and the codes match."""
+    import torch
+    from python_amd.generators import uneven
+    batch = uneven(scale=0.06)        # 600 cells: 30 each of n = 200 and n = 256
+    plan = hip.Plan.for_batch(batch, mode="mixed", graph=True)
+    A, B = hip.to_device(batch)
+    X = torch.zeros_like(B)
+    codes = torch.empty(batch.ncells, dtype=torch.int32, device=B.device)
+    plan.autotune(A, B, X, codes)
+    desc = plan.describe()
+    routed = [g for g in desc["groups"] if g["direct_fp64"]]
+    print("classes routed to FP64:", [(g["kernel"], g["nmax"], g["count"]) for g in routed], flush=True)
+    Xo, co = oracle_lib.solve_batch(batch.n, batch.offA, batch.offV, batch.A, batch.B, nthreads=8)
+    for rep in range(3):
+        X.zero_()
+        plan.solve(A, B, X, codes, stats=True)
+        torch.cuda.synchronize()
+        Xh, ch = X.cpu().numpy(), codes.cpu().numpy()
+        assert np.array_equal(ch, co), rep
+        err = _relerr(batch, Xh, Xo, co)
+        assert err.max() <= TOL, (rep, err.max())
+        for g in routed:
+            for c in np.where((batch.n == g["nmax"]) & (co != 4))[0]:
+                o, n = batch.offV[c], batch.n[c]
+                assert np.array_equal(Xh[o:o + n], Xo[o:o + n]), (rep, c)
