@@ -1,16 +1,26 @@
 #!/usr/bin/env python3
 """Throughput of the per-cell dense solve on one MI355X, on the 10,000-cell uneven batch.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-plain] [--roofline-only]
 
 One "step" = one solve of the whole batch: 10,000 ionization-balance cells, n in
 {12, 30, 64, 96, 128, 200, 256} with counts 3000/2500/1500/1000/1000/500/500, seed 2, inputs
 resident in HBM.  ``value`` is the batched mixed-precision solver (equilibrated FP32 LU + FP64
-refinement + FP64 re-solve of handed-back cells, size classes routed by the autotuner) in solves
-per second; the line also carries the batched FP64 GSL-order solver, the plain per-matrix FP64
-launches (one launch per cell, as the reference's per-call GPU path), the per-cell error against
-the FP64 oracle, the roofline of the dominant kernel and the CPU baseline (the oracle, one worker
-per host core).
+refinement + FP64 re-solve of handed-back cells, size classes routed by the autotuner, one graph
+launch per step) in solves per second; the line also carries the batched FP64 GSL-order solver,
+the plain per-matrix FP64 launches (one launch per cell, as the reference's per-call GPU path),
+the per-cell error against the FP64 oracle, the roofline of the dominant kernel and the CPU
+baseline (the oracle, one worker per host core).
+
+Roofline: the dominant kernel is the kernel instantiation with the largest total time over the
+plan's size classes.  Its launches are timed in isolation (HIP events on the plan's stream,
+``hipla_plan_time_groups``), ``achieved`` = the algorithmic bytes / flops of the cells those
+launches solve divided by their total time, and ``kernel_avg_ms`` the mean launch duration.
+``--roofline-only`` runs just the plan set-up and this measurement (plus one elementwise kernel
+that reads and writes 512 MiB, to calibrate HBM byte counters) and prints the roofline object,
+for profiler runs: the dominant kernel is then launched only by the timed launches.  ``traffic`` is
+read from profiles/traffic.json when it names the same kernel (rocprofv3 FETCH_SIZE +
+WRITE_SIZE per launch, scaled by the copy calibration), else null.
 """
 from __future__ import annotations
 
@@ -37,6 +47,56 @@ def _cores():
         return os.cpu_count() or 1
 
 
+def measure_roofline(plan, A, B, reps=10):
+    """Dominant kernel of an FP64-order plan, from isolated launches of its size classes.
+
+    Every size class of ``plan`` is launched ``reps`` times on its own (HIP events on the plan's
+    stream, nothing else launched); the kernel instantiation with the largest total time is the
+    dominant one.  Returns (roofline dict, per-class list)."""
+    import torch
+    X = torch.empty_like(B)
+    codes = torch.empty(plan.ncells, dtype=torch.int32, device=B.device)
+    desc = plan.describe()
+    gms = plan.time_groups(A, B, X, codes, reps=reps, classes_only=True)
+    by_kernel = {}
+    classes = []
+    for g, t in zip(desc["groups"], gms):
+        name = g["fp64_kernel"]
+        k = by_kernel.setdefault(name, {"ms": 0.0, "bytes": 0.0, "flops": 0.0, "launches": 0, "classes": []})
+        k["ms"] += float(t)
+        k["bytes"] += float(g["bytes"])
+        k["flops"] += float(g["flops"])
+        k["launches"] += 1
+        k["classes"].append(f"n<={g['nmax']} x{g['count']}")
+        classes.append({"kernel": name, "nmax": g["nmax"], "count": g["count"], "ms": round(float(t), 4)})
+    dom = max(by_kernel, key=lambda nm: by_kernel[nm]["ms"])
+    dk = by_kernel[dom]
+    bw = dk["bytes"] / (dk["ms"] * 1e-3) / 1e9
+    fl = dk["flops"] / (dk["ms"] * 1e-3) / 1e12
+    if bw / HBM_PEAK_GBS >= fl / FP64_PEAK_TFLOPS:
+        roof = {"bound": "hbm", "achieved": round(bw, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(bw / HBM_PEAK_GBS, 5)}
+    else:
+        roof = {"bound": "mfma", "achieved": round(fl, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(fl / FP64_PEAK_TFLOPS, 5)}
+    roof["traffic"] = None
+    roof["kernel"] = dom
+    roof["kernel_avg_ms"] = round(dk["ms"] / dk["launches"], 4)
+    roof["launches_per_step"] = dk["launches"]
+    roof["size_classes"] = dk["classes"]
+    roof["algorithmic_bytes_per_launch"] = round(dk["bytes"] / dk["launches"])
+    roof["algorithmic_flops_per_launch"] = round(dk["flops"] / dk["launches"])
+    roof["hbm_achieved_gbs"] = round(bw, 2)
+    roof["fp64_achieved_tflops"] = round(fl, 4)
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        if tr.get("kernel") == dom:
+            roof["traffic"] = tr.get("hbm_bytes_per_launch")
+    return roof, classes
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -45,6 +105,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--quick", action="store_true", help="1/10 of the batch (development only)")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="plan set-up, graph timing and the dominant-kernel measurement only (profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs on a single MI355X (the cell batch would shard "
@@ -62,6 +124,20 @@ def main():
     dev = torch.device("cuda:0")
     A, B = pa.to_device(batch, dev)
     torch.cuda.synchronize()
+
+    if args.roofline_only:
+        # only the isolated size-class launches of the FP64 plan, then one elementwise kernel that
+        # reads and writes 512 MiB (the calibration for HBM byte counters); prints the roofline
+        # object alone
+        fp64 = pa.Plan.for_batch(batch, mode="fp64", graph=False)
+        roof, classes = measure_roofline(fp64, A, B)
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)      # one elementwise kernel: reads and writes 512 MiB
+        torch.cuda.synchronize()
+        print(json.dumps({"roofline": roof, "classes": classes, "calibration_copy_bytes": src.numel() * 8}),
+              flush=True)
+        return 0
 
     def timed(plan, steps, warmup, stats=False):
         X = torch.empty_like(B)
@@ -106,60 +182,44 @@ def main():
     import oracle
     oracle.build()
     cores = _cores()
-    t0 = time.perf_counter()
     Xo, co = oracle.solve_batch(batch.n, batch.offA, batch.offV, batch.A, batch.B, nthreads=cores)
-    cpu_s = time.perf_counter() - t0
-    Xm_h, Xf_h = Xm.cpu().numpy(), Xf.cpu().numpy()
+    Xm_h = Xm.cpu().numpy()
+    Xf_h = Xf.cpu().numpy() if ms_fp64 is not None else None
     stats = pa.stats_to_numpy(sm)
     rel = np.zeros(batch.ncells)
     rel64 = np.zeros(batch.ncells)
-    cm_h, cf_h = cm.cpu().numpy(), cf.cpu().numpy()
+    cm_h = cm.cpu().numpy()
     for c in range(batch.ncells):
         if co[c] == 4:          # singular in FP64: x is not written (solve_matrix code 4)
             continue
         o, n = batch.offV[c], batch.n[c]
         xo = Xo[o:o + n]
-        s = max(np.abs(xo).max(), 1e-300)
-        rel[c] = np.abs(Xm_h[o:o + n] - xo).max() / s
-        rel64[c] = np.abs(Xf_h[o:o + n] - xo).max() / s
+        sc = max(np.abs(xo).max(), 1e-300)
+        rel[c] = np.abs(Xm_h[o:o + n] - xo).max() / sc
+        if Xf_h is not None:
+            rel64[c] = np.abs(Xf_h[o:o + n] - xo).max() / sc
     fb = stats["fallback"]
     steps_hist = np.bincount(stats["refine_steps"][fb == 0], minlength=1).tolist()
 
-    # ---- roofline of the dominant kernel group (HIP events on the plan's stream) ----
+    # ---- per-class times of the mixed plan (isolated launches) ----
     desc = mixed.describe()
-    gms = mixed.time_groups(A, B, Xt, ct, reps=5)
     groups = desc["groups"]
-    dom = int(np.argmax(gms[:len(groups)]))
-    g = groups[dom]
-    g_bytes = float(g["bytes"])
-    g_flops = float(g["flops"])
-    g_ms = float(gms[dom])
-    fp64_group = bool(g["direct_fp64"]) or g["kernel"] == "fp64_over_256"
-    peak_flops = FP64_PEAK_TFLOPS if fp64_group else FP32_PEAK_TFLOPS
-    bw = g_bytes / (g_ms * 1e-3) / 1e9
-    fl = g_flops / (g_ms * 1e-3) / 1e12
-    if bw / HBM_PEAK_GBS >= fl / peak_flops:
-        roof = {"bound": "hbm", "achieved": round(bw, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(bw / HBM_PEAK_GBS, 5)}
-    else:
-        roof = {"bound": "mfma", "achieved": round(fl, 4), "peak": peak_flops, "unit": "TFLOP/s",
-                "frac": round(fl / peak_flops, 5)}
-    roof["traffic"] = None
-    roof["kernel"] = f"{g['kernel']} param={g['param']} nmax={g['nmax']} cells={g['count']}" + \
-        (" (FP64 GSL-order kernel)" if fp64_group else "")
-    roof["kernel_ms"] = round(g_ms, 4)
+    gms = mixed.time_groups(A, B, Xt, ct, reps=5)
+
+    # ---- roofline of the dominant kernel (isolated launches of the FP64 plan's size classes) ----
+    roof, classes = measure_roofline(fp64, A, B)
 
     # ---- CPU baseline: the oracle on a bounded sample (the whole batch, repeated to >= 10 s) ----
     cpu = None
     if not args.no_cpu:
-        reps, total = 0, 0.0
-        while total < 10.0 and reps < 200:
+        reps_c, total = 0, 0.0
+        while total < 10.0 and reps_c < 200:
             t0 = time.perf_counter()
             oracle.solve_batch(batch.n, batch.offA, batch.offV, batch.A, batch.B, nthreads=cores)
             total += time.perf_counter() - t0
-            reps += 1
-        cpu = {"value": round(reps * batch.ncells / total, 1), "unit": "solves/s", "cores": cores,
-               "kind": "port", "sample": f"the full {batch.ncells}-cell batch x {reps} (GSL-order C restatement, "
+            reps_c += 1
+        cpu = {"value": round(reps_c * batch.ncells / total, 1), "unit": "solves/s", "cores": cores,
+               "kind": "port", "sample": f"the full {batch.ncells}-cell batch x {reps_c} (GSL-order C restatement, "
                                          f"one OpenMP worker per core, {total:.1f} s)"}
 
     value = batch.ncells / (ms_mixed * 1e-3)
@@ -174,7 +234,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 LU + fp64 refinement (fp64 GSL-order where routed)",
+        "dtype": "f64" if int((fb == 2).sum()) == batch.ncells else "f32 LU + f64 refinement / f64",
         "data": "synthetic (seeded ionization-balance generator, seed 2)",
         "config": {"workload": "uneven_10k: 10000 cells, n in {12,30,64,96,128,200,256}, counts "
                                "3000/2500/1500/1000/1000/500/500, 1 RHS each, FP64 in/out",
@@ -186,16 +246,17 @@ def main():
         "fallback_rate": round(float((fb == 1).sum()) / batch.ncells, 4),
         "fp64_direct_cells": int((fb == 2).sum()),
         "refine_steps_hist": steps_hist,
-        "batched_fp64": {"solves_per_s": round(batch.ncells / (ms_fp64 * 1e-3), 1), "ms_per_step": round(ms_fp64, 4),
-                         "max_rel_err_vs_oracle": float(rel64.max()),
-                         "codes_equal_oracle": bool(np.array_equal(cf_h, co))},
+        "batched_fp64": None if ms_fp64 is None else
+        {"solves_per_s": round(batch.ncells / (ms_fp64 * 1e-3), 1), "ms_per_step": round(ms_fp64, 4),
+         "max_rel_err_vs_oracle": float(rel64.max()), "codes_equal_oracle": bool(np.array_equal(cf.cpu().numpy(), co))},
         "plain_fp64_per_matrix": None if ms_plain is None else
         {"solves_per_s": round(batch.ncells / (ms_plain * 1e-3), 1), "ms_per_step": round(ms_plain, 3)},
         "speedup_vs_plain_fp64": None if ms_plain is None else round(ms_plain / ms_mixed, 2),
         "roofline": roof,
         "cpu_baseline": cpu,
-        "groups": [{k: g2[k] for k in ("kernel", "param", "nmax", "count", "direct_fp64")} | {"ms": round(float(t), 4)}
-                   for g2, t in zip(groups, gms)],
+        "groups": [{k: g2[k] for k in ("kernel", "fp64_kernel", "nmax", "count", "direct_fp64")} |
+                   {"ms": round(float(t), 4)} for g2, t in zip(groups, gms)],
+        "fp64_classes": classes,
         "fallback_pass_ms": round(float(gms[len(groups)]), 4),
     }
     print(json.dumps(out), flush=True)

@@ -117,8 +117,9 @@ int hipla_plan_autotune (hipla_plan plan, const double *dA, const double *dB, do
 
 /* Profiling helper (no reference counterpart): launch each size-class group of the plan alone
  * `reps` times on the plan's own stream, bracketed by HIP events, and write the average ms per
- * launch into ms_out[0 .. ngroups-1]; ms_out[ngroups] = the FP64 re-solve pass of a mixed plan
- * (0 for FP64 plans), ms_out[ngroups+1] = a whole hipla_solve.  nout >= ngroups + 2. */
+ * launch into ms_out[0 .. ngroups-1].  With nout >= ngroups + 2 also ms_out[ngroups] = the FP64
+ * re-solve pass of a mixed plan (0 for FP64 plans) and ms_out[ngroups+1] = a whole hipla_solve;
+ * with nout == ngroups only the size classes are launched.  nout >= ngroups. */
 int hipla_plan_time_groups (hipla_plan plan, const double *dA, const double *dB, double *dX, int *dcodes,
                             hipla_cell_stats *dstats, int reps, double *ms_out, int nout);
 

@@ -22,3 +22,4 @@ hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, double *w
                                hipStream_t st);
 hipError_t setup_solve_f64_attrs (size_t max_smem);
 int lds_f64_limit ();   // largest n the LDS-resident FP64 kernel holds on this device
+void fp64_kernel_symbol (int nmax, bool keep, char *buf, size_t len);

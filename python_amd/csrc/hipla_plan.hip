@@ -296,13 +296,18 @@ extern "C" int hipla_plan_describe (hipla_plan p, char *buf, int buflen)
       bytes += 8.0 * (nn * nn + 2.0 * nn);
       flops += 2.0 / 3.0 * nn * nn * nn + 2.0 * nn * nn;
     }
-    snprintf (line, sizeof line,
+    // the FP64 kernel instantiation that runs the group (FP64 plans, or a mixed size class
+    // routed straight to FP64)
+    char sym[96];
+    fp64_kernel_symbol (g.nmax, (p->flags & HIPLA_FLAG_KEEP_FACTORS) != 0, sym, sizeof sym);
+    char line2[512];
+    snprintf (line2, sizeof line2,
               "%s{\"kernel\": \"%s\", \"param\": %d, \"nmax\": %d, \"count\": %d, \"grid\": %d, "
               "\"direct_fp64\": %d, \"t_mixed_ms\": %.4f, \"t_direct_ms\": %.4f, \"bytes\": %.0f, "
-              "\"flops\": %.0f}",
+              "\"flops\": %.0f, \"fp64_kernel\": \"%s\"}",
               i ? ", " : "", kind_name (g.kind), g.param, g.nmax, g.count, g.grid, g.direct, g.t_mixed, g.t_direct,
-              bytes, flops);
-    s += line;
+              bytes, flops, sym);
+    s += line2;
   }
   s += "]}";
   snprintf (buf, (size_t) buflen, "%s", s.c_str ());

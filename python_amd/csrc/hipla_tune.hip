@@ -147,9 +147,9 @@ extern "C" int hipla_plan_time_groups (hipla_plan p, const double *dA, const dou
     return 5;
   }
   const int ng = (int) p->groups.size ();
-  if (nout < ng + 2)
+  if (nout < ng)
   {
-    hipla_set_detail ("hipla_plan_time_groups: need %d outputs", ng + 2);
+    hipla_set_detail ("hipla_plan_time_groups: need at least %d outputs", ng);
     return 5;
   }
   hipStream_t st = p->aux.back ();
@@ -157,6 +157,8 @@ extern "C" int hipla_plan_time_groups (hipla_plan p, const double *dA, const dou
   for (int gi = 0; gi < ng; gi++)
     if ((rc = time_group (p, gi, dA, dB, dX, dcodes, dstats, reps, st, ms_out[gi])))
       return rc;
+  if (nout < ng + 2)
+    return 0;        // size classes only
   ms_out[ng] = 0.0;
   if (p->mode == HIPLA_MODE_MIXED)
   {

@@ -1,0 +1,127 @@
+"""Summarise the rocprofv3 output of python_amd/profile_bench.sh into profiles/.
+
+    python -m python_amd.profile_summary [OUT_DIR]      (default gpurun_out/rocprof)
+
+Reads the kernel statistics and kernel trace of `bench.py --roofline-only --no-cpu`, its
+FETCH_SIZE and WRITE_SIZE counter runs and the roofline object the command printed, and writes
+  profiles/<kernel>_uneven10k_trace.txt   per-kernel statistics, the dominant kernel's dispatches
+                                          against the HIP-event average reported by bench.py
+  profiles/<kernel>_uneven10k_hbm.txt     HBM bytes per dispatch, calibrated on a 512 MiB copy
+  profiles/traffic.json                   the calibrated bytes per launch bench.py reports
+"""
+from __future__ import annotations
+
+import csv
+import json
+import os
+import re
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CMD = ("python_amd/profile_bench.sh  (rocprofv3 --kernel-trace --stats --output-format csv -- python3 bench.py "
+       "--roofline-only --no-cpu; then the same command under --pmc FETCH_SIZE and under --pmc WRITE_SIZE)")
+
+
+def _short(name: str) -> str:
+    m = re.match(r"(?:void )?(\w+)<([^>]*)>", name)
+    if not m:
+        return re.sub(r"\W+", "_", name)[:40]
+    return m.group(1) + "_" + re.sub(r"[^0-9a-z]+", "_", m.group(2).lower()).strip("_")
+
+
+def _roofline(out):
+    with open(os.path.join(out, "trace.log")) as f:
+        for line in f:
+            if line.startswith('{"roofline"'):
+                return json.loads(line)
+    raise RuntimeError("no roofline line in trace.log")
+
+
+def _counters(path, counter):
+    """kernel name -> list of per-dispatch values of one counter."""
+    per = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] != counter:
+                continue
+            per.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    return per
+
+
+def main(out=None):
+    out = out or os.path.join(ROOT, "gpurun_out", "rocprof")
+    rl = _roofline(out)
+    roof = rl["roofline"]
+    dom = roof["kernel"]
+    calib_bytes = rl["calibration_copy_bytes"]
+
+    stats = list(csv.DictReader(open(os.path.join(out, "trace", "bench_kernel_stats.csv"))))
+    trace = list(csv.DictReader(open(os.path.join(out, "trace", "bench_kernel_trace.csv"))))
+    dom_d = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in trace if r["Kernel_Name"].startswith("void " + dom)
+             or r["Kernel_Name"].startswith(dom)]
+    os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
+    short = _short(dom)
+
+    lines = [f"# Kernel trace of the bench workload (uneven 10k-cell batch, FP64 size classes launched in isolation)",
+             f"command: {CMD}", "",
+             f"{'kernel':78s} {'calls':>6s} {'avg us':>10s} {'total ms':>10s} {'%':>6s}"]
+    for r in stats[:14]:
+        lines.append(f"{r['Name'][:78]:78s} {int(r['Calls']):6d} {float(r['AverageNs']) / 1e3:10.1f} "
+                     f"{float(r['TotalDurationNs']) / 1e6:10.3f} {float(r['Percentage']):6.2f}")
+    avg_ms = sum(dom_d) / len(dom_d) / 1e6 if dom_d else float("nan")
+    lines += ["", f"dominant kernel: {dom}",
+              f"  dispatches in the trace: {len(dom_d)} (size classes {', '.join(roof['size_classes'])}, "
+              f"{roof['launches_per_step']} per repetition)",
+              f"  rocprofv3 average duration: {avg_ms:.4f} ms",
+              f"  bench.py HIP-event average: {roof['kernel_avg_ms']:.4f} ms per launch",
+              f"  algorithmic bytes per launch: {roof['algorithmic_bytes_per_launch']}, flops per launch: "
+              f"{roof['algorithmic_flops_per_launch']}",
+              f"  achieved: {roof['achieved']} {roof['unit']} of {roof['peak']} ({roof['bound']} bound), "
+              f"frac {roof['frac']}; HBM {roof['hbm_achieved_gbs']} GB/s algorithmic"]
+    with open(os.path.join(ROOT, "profiles", f"{short}_uneven10k_trace.txt"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+    fetch = _counters(os.path.join(out, "fetch", "bench_counter_collection.csv"), "FETCH_SIZE")
+    write = _counters(os.path.join(out, "write", "bench_counter_collection.csv"), "WRITE_SIZE")
+
+    def pick(per, pat):
+        for k, v in per.items():
+            if pat(k):
+                return k, v
+        raise RuntimeError("kernel not found")
+
+    _, f_dom = pick(fetch, lambda k: dom in k)
+    _, w_dom = pick(write, lambda k: dom in k)
+    cname, f_cal = pick(fetch, lambda k: "MulFunctor" in k)
+    _, w_cal = pick(write, lambda k: "MulFunctor" in k)
+    kib = calib_bytes / 1024.0
+    ff = max(f_cal) / kib
+    fw = max(w_cal) / kib
+    fetch_b = sum(f_dom) / len(f_dom) * 1024.0 / ff
+    write_b = sum(w_dom) / len(w_dom) * 1024.0 / fw
+    traffic = fetch_b + write_b
+    tj = {"kernel": dom, "hbm_bytes_per_launch": round(traffic), "fetch_bytes_per_launch": round(fetch_b),
+          "write_bytes_per_launch": round(write_b), "algorithmic_bytes_per_launch": roof["algorithmic_bytes_per_launch"],
+          "fetch_kib_per_dispatch_raw": round(sum(f_dom) / len(f_dom), 1),
+          "write_kib_per_dispatch_raw": round(sum(w_dom) / len(w_dom), 1),
+          "calibration": {"kernel": cname[:90], "bytes_read_and_written": calib_bytes,
+                          "fetch_factor": round(ff, 4), "write_factor": round(fw, 4)},
+          "command": CMD}
+    with open(os.path.join(ROOT, "profiles", "traffic.json"), "w") as f:
+        json.dump(tj, f, indent=1)
+        f.write("\n")
+    hl = [f"# HBM traffic of {dom} on the bench workload (per dispatch, {len(f_dom)} dispatches)",
+          f"command: {CMD}", "",
+          f"calibration: {cname[:90]} reads and writes {calib_bytes} B; FETCH_SIZE reads {max(f_cal):.0f} KiB "
+          f"(factor {ff:.4f}), WRITE_SIZE {max(w_cal):.0f} KiB (factor {fw:.4f})",
+          f"FETCH_SIZE {sum(f_dom) / len(f_dom):.1f} KiB per dispatch -> {fetch_b / 1e6:.2f} MB",
+          f"WRITE_SIZE {sum(w_dom) / len(w_dom):.1f} KiB per dispatch -> {write_b / 1e6:.2f} MB",
+          f"HBM traffic per launch: {traffic / 1e6:.2f} MB against {roof['algorithmic_bytes_per_launch'] / 1e6:.2f} MB "
+          f"algorithmic ({traffic / roof['algorithmic_bytes_per_launch']:.2f}x)"]
+    with open(os.path.join(ROOT, "profiles", f"{short}_uneven10k_hbm.txt"), "w") as f:
+        f.write("\n".join(hl) + "\n")
+    print("\n".join(lines + [""] + hl))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else None)

@@ -178,10 +178,11 @@ class Plan:
               "hipla_adjoint")
         return Lam, GradA
 
-    def time_groups(self, A, B, X, codes, stats=None, reps: int = 5):
-        """Per-group average ms (HIP events on the plan's stream): (groups, fallback pass, whole)."""
+    def time_groups(self, A, B, X, codes, stats=None, reps: int = 5, classes_only: bool = False):
+        """Per-group average ms (HIP events on the plan's stream): (groups, fallback pass, whole),
+        or the groups alone with ``classes_only`` (nothing else is launched then)."""
         ng = len(self.describe()["groups"])
-        out = np.zeros(ng + 2)
+        out = np.zeros(ng if classes_only else ng + 2)
         check(lib().hipla_plan_time_groups(self._h, _ptr(A), _ptr(B), _ptr(X), _ptr(codes), _ptr(stats), int(reps),
                                            out.ctypes.data_as(_lib._dp), len(out)), "hipla_plan_time_groups")
         return out
