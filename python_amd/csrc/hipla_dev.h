@@ -31,6 +31,11 @@ struct SolveArgs
   // optional device-side list length (FP64 re-solve of cells the mixed solver hands back):
   // the kernels then process min(*dcount, nlist) cells, persistent over a fixed grid
   const int *dcount;
+  // factor store (A's layout: the factored matrix in GSL's row order) and forward-substituted
+  // right-hand side (b's layout), written by the workgroup factor kernels and read by the
+  // back-substitution kernel; with HIPLA_FLAG_KEEP_FACTORS, F is the kept LU store
+  double *F;
+  double *Y;
 };
 
 __device__ __forceinline__ int list_len (const SolveArgs &s)

@@ -14,12 +14,15 @@
 
 // ---- FP64, GSL operation order (k_solve_f64.hip) ----
 size_t wave_f64_smem (int nc, int wpb, bool keep);
-size_t lds_f64_smem (int n);
 size_t blocked_f64_smem (int nmax);
 hipError_t launch_wave_f64 (const SolveArgs &a, int nc, bool keep, int grid, hipStream_t st);
-hipError_t launch_lds_f64 (const SolveArgs &a, int nmax, bool keep, int grid, hipStream_t st);
-hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, double *work, int64_t wstride, bool keep,
-                               hipStream_t st);
+// workgroup-resident route (64 < n <= wgr_f64_limit ()): register-tiled factorization
+// (k_tile.hip) + back substitution
+hipError_t launch_wgr_f64 (const SolveArgs &a, int nmax, bool keep, int grid, hipStream_t st);
+hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, bool keep, hipStream_t st);
 hipError_t setup_solve_f64_attrs (size_t max_smem);
-int lds_f64_limit ();   // largest n the LDS-resident FP64 kernel holds on this device
+int wgr_f64_limit ();   // largest n of the workgroup-resident FP64 route
+int tile_f64_limit ();
+hipError_t launch_tile_f64 (const SolveArgs &a, int nmax, bool keep, int grid, hipStream_t st);
+void tile_kernel_symbol (int nmax, bool keep, char *buf, size_t len);
 void fp64_kernel_symbol (int nmax, bool keep, char *buf, size_t len);

@@ -35,8 +35,8 @@ void hipla_set_detail (const char *fmt, ...);
 enum KernelKind
 {
   KK_WAVE_F64 = 0,     // k_wave_f64<NC>            one wave per matrix, FP64 GSL order
-  KK_LDS_F64 = 1,      // k_lds_f64                 workgroup per matrix, [A|b] in LDS
-  KK_BLOCKED_F64 = 2,  // k_blocked_f64             blocked, panel in LDS, HBM workspace
+  KK_WGR_F64 = 1,      // k_tile_f64 + k_bsub_f64   workgroup per matrix, [A|b] in registers
+  KK_BLOCKED_F64 = 2,  // k_blocked_f64 + k_bsub_f64  blocked, panel in LDS, factors in place
   KK_WAVE_MIX = 3,     // k_wave_mixed<NC>          FP32 LU in registers + FP64 refinement
   KK_LDS_MIX = 4,      // k_lds_mixed               FP32 LU in LDS + FP64 refinement
   KK_BLOCKED_MIX = 5,  // k_blocked_mixed           FP32 blocked LU in HBM workspace + refinement
@@ -52,8 +52,6 @@ struct Group
   int first = 0;      // offset into the plan's cell list
   int count = 0;
   int grid = 0;       // persistent grid size (blocked kernels)
-  int wslot0 = 0;     // first workspace slot of this group (groups that may run concurrently
-                      // with another blocked launch get disjoint slot ranges)
   int stream = 0;     // fork-join branch
   int direct = 0;     // mixed plans: 1 = the tuner routes this size class straight to FP64
   double t_mixed = -1.0, t_direct = -1.0;   // tuner measurements (ms), -1 if not tuned
@@ -76,9 +74,6 @@ struct hipla_plan_s
   int64_t *d_offA = nullptr, *d_offV = nullptr;
   int *d_list = nullptr;
   // workspaces
-  double *d_work = nullptr;   // blocked kernels: per-workgroup slot
-  int64_t wstride = 0;
-  int work_slots = 0;
   int *d_fallback = nullptr;  // mixed: 5 lists [count, cells...] of cells to re-solve in FP64
   int fb_route_used[5] = { 0, 0, 0, 0, 0 };
   int *d_list_desc = nullptr;  // every cell, largest order first (inverse launches)
@@ -87,6 +82,9 @@ struct hipla_plan_s
   int inv_slots = 0;
   double *d_LU = nullptr;     // keep-factors store (A's layout)
   int *d_piv = nullptr;       // keep-factors pivots (B's layout)
+  double *d_F = nullptr;      // factor store of the workgroup kernels (d_LU when factors are kept)
+  bool own_F = false;
+  double *d_Y = nullptr;      // forward-substituted right-hand sides (B's layout)
   // fork-join streams / events
   std::vector<hipStream_t> aux;
   std::vector<hipEvent_t> ev;

@@ -25,8 +25,8 @@ static const char *kind_name (int k)
   {
   case KK_WAVE_F64:
     return "wave_f64";
-  case KK_LDS_F64:
-    return "lds_f64";
+  case KK_WGR_F64:
+    return "tile_f64";
   case KK_BLOCKED_F64:
     return "blocked_f64";
   case KK_WAVE_MIX:
@@ -57,9 +57,9 @@ static void route_fp64 (int n, int &kind, int &param)
     kind = KK_WAVE_F64;
     param = 64;
   }
-  else if (n <= lds_f64_limit ())
+  else if (n <= wgr_f64_limit ())
   {
-    kind = KK_LDS_F64;
+    kind = KK_WGR_F64;
     param = 0;
   }
   else
@@ -77,7 +77,8 @@ static int free_plan_device (hipla_plan p)
     (void) hipGraphDestroy (p->graph);
   p->gexec = nullptr;
   p->graph = nullptr;
-  void *bufs[] = { p->d_n, p->d_offA, p->d_offV, p->d_list, p->d_list_desc, p->d_inv_work, p->d_work, p->d_fallback, p->d_LU, p->d_piv };
+  void *bufs[] = { p->d_n,   p->d_offA, p->d_offV, p->d_list, p->d_list_desc, p->d_inv_work, p->d_fallback,
+                   p->d_LU,  p->d_piv,  p->own_F ? p->d_F : nullptr, p->d_Y };
   for (void *b : bufs)
     if (b)
       (void) hipFree (b);
@@ -156,8 +157,6 @@ extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, con
   });
   p->list = idx;
   const DeviceInfo &dev = hipla_device_info ();
-  int64_t wneed = 0;
-  int wslots = 0, wcursor = 0;
   for (int i = 0; i < ncells;)
   {
     int j = i;
@@ -170,43 +169,26 @@ extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, con
     g.count = j - i;
     g.nmax = n[idx[i]];
     g.stream = (int) p->groups.size ();
-    // groups that run (or, in mixed plans, may be routed to) the blocked FP64 kernel: the
-    // groups of one solve run concurrently, so each gets its own range of workspace slots
+    // persistent grid of groups that run (or, in mixed plans, may be routed to) the blocked
+    // FP64 kernel; it factors each cell in place in the plan's factor store
     if (g.kind == KK_BLOCKED_F64 || g.kind == KK_BLOCKED_MIX ||
-        (mode == HIPLA_MODE_MIXED && g.nmax > lds_f64_limit ()))
-    {
+        (mode == HIPLA_MODE_MIXED && g.nmax > wgr_f64_limit ()))
       g.grid = std::min (g.count, 2 * dev.num_cu);
-      g.wslot0 = wcursor;
-      wcursor += g.grid;
-      wneed = std::max<int64_t> (wneed, (int64_t) g.nmax * g.nmax + g.nmax);
-      wslots = std::max (wslots, wcursor);
-    }
     p->groups.push_back (g);
     i = j;
   }
 
-  // mixed mode: FP64 routes the fallback re-solve can need (the blocked one needs a workspace)
+  // mixed mode: FP64 routes the fallback re-solve can need
   if (mode == HIPLA_MODE_MIXED)
   {
-    int nblk = 0, nbmax = 0;
     for (int c = 0; c < ncells; c++)
     {
       if (kind[c] == KK_BLOCKED_MIX)
         continue;
       int fk, fp;
       route_fp64 (n[c], fk, fp);
-      const int r = fk == KK_WAVE_F64 ? (fp == 16 ? 0 : fp == 32 ? 1 : 2) : fk == KK_LDS_F64 ? 3 : 4;
+      const int r = fk == KK_WAVE_F64 ? (fp == 16 ? 0 : fp == 32 ? 1 : 2) : fk == KK_WGR_F64 ? 3 : 4;
       p->fb_route_used[r] = 1;
-      if (r == 4)
-      {
-        nblk++;
-        nbmax = std::max (nbmax, n[c]);
-      }
-    }
-    if (nblk)
-    {
-      wneed = std::max<int64_t> (wneed, (int64_t) nbmax * nbmax + nbmax);
-      wslots = std::max (wslots, std::min (nblk, dev.num_cu));
     }
   }
 
@@ -238,18 +220,30 @@ extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, con
     PCHECK (hipMalloc (&p->d_list_desc, sizeof (int) * ncells));
     PCHECK (hipMemcpy (p->d_list_desc, desc.data (), sizeof (int) * ncells, hipMemcpyHostToDevice));
   }
-  if (wslots > 0)
-  {
-    p->wstride = (wneed + 63) & ~(int64_t) 63;
-    p->work_slots = wslots;
-    PCHECK (hipMalloc (&p->d_work, sizeof (double) * p->wstride * wslots));
-  }
   if (mode == HIPLA_MODE_MIXED)
     PCHECK (hipMalloc (&p->d_fallback, sizeof (int) * 5 * (ncells + 1)));
   if (flags & HIPLA_FLAG_KEEP_FACTORS)
   {
     PCHECK (hipMalloc (&p->d_LU, sizeof (double) * p->totA));
     PCHECK (hipMalloc (&p->d_piv, sizeof (int) * p->totV));
+  }
+  // factor and y stores of the workgroup factor kernels (cells above 64), read by the
+  // back-substitution kernel; kept factors double as the factor store
+  {
+    int nbig = 0;
+    for (int c = 0; c < ncells; c++)
+      nbig |= n[c] > 64;
+    if (nbig)
+    {
+      if (p->d_LU)
+        p->d_F = p->d_LU;
+      else
+      {
+        PCHECK (hipMalloc (&p->d_F, sizeof (double) * p->totA));
+        p->own_F = true;
+      }
+      PCHECK (hipMalloc (&p->d_Y, sizeof (double) * p->totV));
+    }
   }
   const int nbranch = (int) p->groups.size ();
   p->aux.resize (nbranch + 1, nullptr);
@@ -333,6 +327,8 @@ static SolveArgs make_args (hipla_plan p, const double *dA, const double *dB, do
   a.LU = p->d_LU;
   a.piv = p->d_piv;
   a.dcount = nullptr;
+  a.F = p->d_F;
+  a.Y = p->d_Y;
   return a;
 }
 
@@ -343,11 +339,10 @@ static hipError_t launch_group_f64 (hipla_plan p, const SolveArgs &a, const Grou
   {
   case KK_WAVE_F64:
     return launch_wave_f64 (a, g.param, keep, 0, st);
-  case KK_LDS_F64:
-    return launch_lds_f64 (a, g.nmax, keep, 0, st);
+  case KK_WGR_F64:
+    return launch_wgr_f64 (a, g.nmax, keep, 0, st);
   case KK_BLOCKED_F64:
-    return launch_blocked_f64 (a, g.nmax, std::min (g.grid, std::max (a.nlist, 1)),
-                               p->d_work + (size_t) g.wslot0 * p->wstride, p->wstride, keep, st);
+    return launch_blocked_f64 (a, g.nmax, std::min (g.grid, std::max (a.nlist, 1)), keep, st);
   default:
     return hipErrorInvalidValue;
   }

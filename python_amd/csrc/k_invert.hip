@@ -348,6 +348,8 @@ int plan_enqueue_invert (hipla_plan p, const double *dA, double *dInv, int *dcod
   a.LU = nullptr;
   a.piv = nullptr;
   a.dcount = nullptr;
+  a.F = p->d_F;
+  a.Y = p->d_Y;
   HIPLA_CHECK (launch_invert_f64 (a, nmax, p->inv_slots, p->d_inv_work, p->inv_wstride, dInv, st));
   return 0;
 }

@@ -922,6 +922,8 @@ static SolveArgs group_args (hipla_plan p, const double *dA, const double *dB, d
   a.LU = nullptr;
   a.piv = nullptr;
   a.dcount = nullptr;
+  a.F = p->d_F;
+  a.Y = p->d_Y;
   return a;
 }
 
@@ -932,11 +934,10 @@ static int launch_group_direct_fp64 (hipla_plan p, const SolveArgs &a, const Gro
 {
   if (g.nmax <= 64)
     HIPLA_CHECK (launch_wave_f64 (a, g.nmax <= 16 ? 16 : g.nmax <= 32 ? 32 : 64, false, 0, s));
-  else if (g.nmax <= lds_f64_limit ())
-    HIPLA_CHECK (launch_lds_f64 (a, g.nmax, false, 0, s));
+  else if (g.nmax <= wgr_f64_limit ())
+    HIPLA_CHECK (launch_wgr_f64 (a, g.nmax, false, 0, s));
   else
-    HIPLA_CHECK (launch_blocked_f64 (a, g.nmax, std::max (1, std::min (g.count, g.grid)),
-                                     p->d_work + (size_t) g.wslot0 * p->wstride, p->wstride, false, s));
+    HIPLA_CHECK (launch_blocked_f64 (a, g.nmax, std::max (1, std::min (g.count, g.grid)), false, s));
   if (dstats)
   {
     hipLaunchKernelGGL (k_mark_fp64_stats, dim3 ((g.count + 255) / 256), dim3 (256), 0, s, a.list, g.count, dstats);
@@ -957,7 +958,7 @@ int enqueue_mixed_group (hipla_plan p, int gi, const double *dA, const double *d
   m.stats = dstats;
   m.fb = p->d_fallback;
   m.fbcap = p->ncells;
-  m.lds_limit = lds_f64_limit ();
+  m.lds_limit = wgr_f64_limit ();
   m.max_refine = p->max_refine;
   m.tol = p->tol;
   m.no_fallback = (p->flags & HIPLA_FLAG_NO_FALLBACK) ? 1 : 0;
@@ -997,13 +998,14 @@ int enqueue_mixed_fallback (hipla_plan p, const double *dA, const double *dB, do
     a.LU = nullptr;
     a.piv = nullptr;
     a.dcount = p->d_fallback + r * (fbcap + 1);
+    a.F = p->d_F;
+    a.Y = p->d_Y;
     if (r < 3)
       HIPLA_CHECK (launch_wave_f64 (a, r == 0 ? 16 : r == 1 ? 32 : 64, false, 2 * dev.num_cu, st));
     else if (r == 3)
-      HIPLA_CHECK (launch_lds_f64 (a, std::min (nmax_all, lds_f64_limit ()), false, dev.num_cu, st));
+      HIPLA_CHECK (launch_wgr_f64 (a, std::min (nmax_all, wgr_f64_limit ()), false, dev.num_cu, st));
     else
-      HIPLA_CHECK (launch_blocked_f64 (a, nmax_all, std::min (p->work_slots, dev.num_cu), p->d_work, p->wstride,
-                                       false, st));
+      HIPLA_CHECK (launch_blocked_f64 (a, nmax_all, dev.num_cu, false, st));
   }
   return 0;
 }

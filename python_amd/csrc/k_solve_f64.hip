@@ -16,16 +16,21 @@
 // (values equal GSL's; -0.0 vs +0.0 and inf*0 in overflowing inputs are the documented
 // differences).
 //
-// Three shapes, chosen per cell by the host plan:
+// Shapes, chosen per cell by the host plan:
 //   k_wave_f64<NC>     n <= NC <= 64: one wave per matrix, lane = row, row in registers; a pivot
 //                      lane keeps its U row in registers (no row swaps: positions tracked per
-//                      lane, so GSL's tie rule uses positions, not lanes).
-//   k_lds_f64          n <= ~141: one workgroup per matrix, [A|b] resident in LDS, positions
-//                      tracked instead of row swaps, one barrier per elimination step.
-//   k_blocked_f64<NB>  larger n: right-looking blocked LU, NB-column panel factored in LDS,
-//                      row permutation, U12 and the A22 update streamed through a per-workgroup
-//                      HBM workspace, persistent over the cell list.
+//                      lane, so GSL's tie rule uses positions, not lanes); factor, solve and
+//                      residual check in one kernel.
+//   k_tile_f64         64 < n <= 175 (k_tile.hip): one workgroup per matrix, [A|b] in registers.
+//   k_blocked_f64<NB>  larger n: right-looking blocked LU, NB-column panel factored in LDS, the
+//                      panel's row permutation, U12 and the A22 update applied in place in the
+//                      factor store, persistent over the cell list.
+// The last two write the factors and the forward-substituted b; k_bsub_f64 (one wave per cell,
+// many cells per CU, so the serial dtrsv chains of different cells overlap) back-substitutes
+// and checks the residual.
 #include <stdio.h>
+
+#include <algorithm>
 
 #include "hipla_dev.h"
 #include "hipla_kernels.h"
@@ -36,54 +41,95 @@ static size_t hipla_device_max_smem ()
   return hipla_device_info ().max_smem;
 }
 
-// tmp - p[j0] - p[j0+1] - ... - p[n-1], subtracted in ascending order (a dependent chain), p
-// in LDS and 16-byte aligned.  Groups of eight products are read while the previous group is
-// being subtracted, so the LDS latency overlaps the chain.
+// A value every lane of the wave holds (an LDS broadcast read), moved to a scalar register.
+__device__ __forceinline__ int uni (int v)
+{
+  return __builtin_amdgcn_readfirstlane (v);
+}
+
+// Row `row` of an n x n row-major matrix (row wave-uniform: a scalar pointer).
+__device__ __forceinline__ double *wrow (double *W, int n, int row)
+{
+  return W + (size_t) (unsigned) row * (unsigned) n;
+}
+
+// Eight consecutive products p[g .. g+7] (g even: 16-byte aligned reads).
+struct Oct
+{
+  double2 a, b, c, d;
+};
+
+__device__ __forceinline__ Oct load_oct (const double *p)
+{
+  Oct o;
+  o.a = *reinterpret_cast<const double2 *> (p);
+  o.b = *reinterpret_cast<const double2 *> (p + 2);
+  o.c = *reinterpret_cast<const double2 *> (p + 4);
+  o.d = *reinterpret_cast<const double2 *> (p + 6);
+  return o;
+}
+
+__device__ __forceinline__ double sub_oct (double t, const Oct &o)
+{
+  t = t - o.a.x;
+  t = t - o.a.y;
+  t = t - o.b.x;
+  t = t - o.b.y;
+  t = t - o.c.x;
+  t = t - o.c.y;
+  t = t - o.d.x;
+  t = t - o.d.y;
+  return t;
+}
+
+// Only the elements g+u with lo <= g+u < hi (wave-uniform bounds).
+__device__ __forceinline__ double sub_oct_masked (double t, const Oct &o, int g, int lo, int hi)
+{
+  const double v[8] = { o.a.x, o.a.y, o.b.x, o.b.y, o.c.x, o.c.y, o.d.x, o.d.y };
+#pragma unroll
+  for (int u = 0; u < 8; u++)
+    if (g + u >= lo && g + u < hi)
+      t = t - v[u];
+  return t;
+}
+
+// tmp - p[j] - p[j+1] - ... - p[n-1], subtracted in ascending order (a dependent chain); p in
+// LDS, 16-byte aligned, and readable up to p[n+6].  The products are read in groups of eight,
+// the next group's reads in flight while the current group is subtracted (two groups per turn
+// of the loop, so the two stay in separate registers).
 __device__ __forceinline__ double chain_sub (double tmp, const double *p, int j, int n)
 {
-  if ((j & 1) && j < n)
+  if (j >= n)
+    return tmp;
+  int g = j & ~1;
+  Oct c = load_oct (p + g);
+  if (g + 8 >= n || g < j)
   {
-    tmp = tmp - p[j];
-    j++;
+    // the first group is partial (or the only one)
+    const Oct d = (g + 8 < n) ? load_oct (p + g + 8) : c;
+    __builtin_amdgcn_sched_barrier (0);
+    tmp = sub_oct_masked (tmp, c, g, j, n);
+    if (g + 8 >= n)
+      return tmp;
+    c = d;
+    g += 8;
   }
-  if (j + 8 <= n)
+  // here c = p[g .. g+7] with g >= j
+  while (true)
   {
-    double2 c0 = *reinterpret_cast<const double2 *> (p + j);
-    double2 c1 = *reinterpret_cast<const double2 *> (p + j + 2);
-    double2 c2 = *reinterpret_cast<const double2 *> (p + j + 4);
-    double2 c3 = *reinterpret_cast<const double2 *> (p + j + 6);
-    j += 8;
-    for (; j + 8 <= n; j += 8)
-    {
-      const double2 d0 = *reinterpret_cast<const double2 *> (p + j);
-      const double2 d1 = *reinterpret_cast<const double2 *> (p + j + 2);
-      const double2 d2 = *reinterpret_cast<const double2 *> (p + j + 4);
-      const double2 d3 = *reinterpret_cast<const double2 *> (p + j + 6);
-      tmp = tmp - c0.x;
-      tmp = tmp - c0.y;
-      tmp = tmp - c1.x;
-      tmp = tmp - c1.y;
-      tmp = tmp - c2.x;
-      tmp = tmp - c2.y;
-      tmp = tmp - c3.x;
-      tmp = tmp - c3.y;
-      c0 = d0;
-      c1 = d1;
-      c2 = d2;
-      c3 = d3;
-    }
-    tmp = tmp - c0.x;
-    tmp = tmp - c0.y;
-    tmp = tmp - c1.x;
-    tmp = tmp - c1.y;
-    tmp = tmp - c2.x;
-    tmp = tmp - c2.y;
-    tmp = tmp - c3.x;
-    tmp = tmp - c3.y;
+    if (g + 8 >= n)
+      return sub_oct_masked (tmp, c, g, g, n);
+    const Oct d = load_oct (p + g + 8);
+    __builtin_amdgcn_sched_barrier (0);
+    tmp = sub_oct (tmp, c);
+    g += 8;
+    if (g + 8 >= n)
+      return sub_oct_masked (tmp, d, g, g, n);
+    c = load_oct (p + g + 8);
+    __builtin_amdgcn_sched_barrier (0);
+    tmp = sub_oct (tmp, d);
+    g += 8;
   }
-  for (; j < n; j++)
-    tmp = tmp - p[j];
-  return tmp;
 }
 
 // Back substitution U x = y in dtrsv order, run by one wave.  Row i of U is at U + r*ldu
@@ -330,223 +376,6 @@ __global__ __launch_bounds__ (64 * WPB, NC == 64 ? 3 : 4) void k_wave_f64 (Solve
 }
 
 // ============================================================================================
-// k_lds_f64: one workgroup per matrix, [A | b] resident in LDS
-// ============================================================================================
-//
-// Rows stay where they were loaded; phys[q] names the row at position q (GSL's row order after
-// its swaps) and prow[k] the pivot row of step k, i.e. the row that ends at position k.  Step k: every thread knows the pivot (row p at position pq) from the previous
-// step's candidates; the pivot row's columns k+1..n go to registers, and the waves eliminate
-// positions k+1..n-1 (the row displaced from position k takes position pq), RU rows per wave
-// at a time.  Lane 0 of each wave also holds column k+1 of every row it updates, so it keeps the
-// wave's candidate for the next pivot (largest |a|, ties to the smallest position) and one
-// barrier per step suffices; the candidate slots alternate between two buffers by step parity.
-template <int BS, bool KEEP>
-__global__ __launch_bounds__ (BS) void k_lds_f64 (SolveArgs s)
-{
-  extern __shared__ double M[];
-  constexpr int NW = BS / 64;
-  constexpr int RU = 4;            // rows per wave per pass
-  constexpr int TC = 3;            // column chunks of 64 per row (n + 1 <= 192)
-  __shared__ long long r_key[2][NW];
-  __shared__ int r_pos[2][NW], r_row[2][NW];
-  __shared__ int s_fail;
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int wv = __builtin_amdgcn_readfirstlane (tid >> 6);   // wave-uniform
-  const int neff = list_len (s);
-  for (int item = blockIdx.x; item < neff; item += gridDim.x)
-  {
-    const int cell = s.list[item];
-    const int n = s.n[cell];
-    const int ld = (n + 1) | 1;
-    double *xs = M + (((size_t) n * ld + 1) & ~(size_t) 1);   // 16-byte aligned
-    int *phys = (int *) (xs + n);
-    int *prow = phys + n;
-    const double *A = s.A + s.offA[cell];
-    const double *b = s.B + s.offV[cell];
-
-    for (int r = wv; r < n; r += NW)
-      for (int c = lane; c < n; c += 64)
-        M[r * ld + c] = A[(size_t) r * n + c];
-    for (int r = tid; r < n; r += BS)
-    {
-      M[r * ld + n] = b[r];
-      phys[r] = r;
-    }
-    if (tid == 0)
-      s_fail = 0x7fffffff;
-    __syncthreads ();
-
-    // candidates for the first pivot: one row per thread (n <= BS)
-    {
-      const long long key = tid < n ? pivot_key (M[tid * ld]) : -1;
-      const int wl = wave_pivot_lane (tid < n, key, tid);
-      if (lane == 0)
-      {
-        r_key[0][wv] = wl >= 0 ? (long long) rl_u64 ((unsigned long long) key, wl) : -1;
-        r_pos[0][wv] = wl >= 0 ? __builtin_amdgcn_readlane (tid, wl) : 0x7fffffff;
-        r_row[0][wv] = r_pos[0][wv];
-      }
-    }
-    __syncthreads ();
-
-    int singular = 0;
-    for (int k = 0; k < n; k++)
-    {
-      const int cb = k & 1, nb = cb ^ 1;
-      int pq, p;
-      combine_pivot_candidates (r_key[cb], r_pos[cb], r_row[cb], NW, lane, pq, p);
-      const int q0 = __builtin_amdgcn_readfirstlane (phys[k]);
-      const double *Mp = M + p * ld;
-      const double piv = Mp[k];
-      singular |= (piv == 0.0);
-      const bool recip = fabs (piv) >= DBL_MIN;
-      const double sc = 1.0 / piv;
-      double ur[TC];
-#pragma unroll
-      for (int t = 0; t < TC; t++)
-      {
-        const int j = k + 1 + lane + 64 * t;
-        ur[t] = j <= n ? Mp[j] : 0.0;
-      }
-      if (KEEP && tid == 0 && s.piv)
-        s.piv[s.offV[cell] + k] = pq;
-
-      long long ck = -1;
-      int cp = 0x7fffffff, cr = 0;
-      for (int qb = k + 1 + wv; qb < n; qb += RU * NW)
-      {
-        int ri[RU];
-        double ak[RU], v[RU][TC];
-#pragma unroll
-        for (int u = 0; u < RU; u++)
-        {
-          const int q = qb + u * NW;
-          ri[u] = q < n ? ((q == pq) ? q0 : __builtin_amdgcn_readfirstlane (phys[q])) : p;
-        }
-#pragma unroll
-        for (int u = 0; u < RU; u++)
-        {
-          const double *Mi = M + ri[u] * ld;
-          ak[u] = Mi[k];
-#pragma unroll
-          for (int t = 0; t < TC; t++)
-          {
-            const int j = k + 1 + lane + 64 * t;
-            v[u][t] = j <= n ? Mi[j] : 0.0;
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < RU; u++)
-        {
-          if (qb + u * NW < n)
-          {
-            double *Mi = M + ri[u] * ld;
-            const double l = recip ? ak[u] * sc : ak[u] / piv;
-#pragma unroll
-            for (int t = 0; t < TC; t++)
-            {
-              const int j = k + 1 + lane + 64 * t;
-              if (j <= n)
-                Mi[j] = v[u][t] - l * ur[t];
-            }
-            if (lane == 0)
-              Mi[k] = l;
-          }
-        }
-        // lane 0 holds column k+1 (t = 0) of each row: the next pivot's candidates
-#pragma unroll
-        for (int u = 0; u < RU; u++)
-        {
-          const int q = qb + u * NW;
-          if (q < n && k + 1 < n)
-          {
-            const double l = recip ? ak[u] * sc : ak[u] / piv;
-            const long long key = pivot_key (v[u][0] - l * ur[0]);
-            if (key > ck || (key == ck && q < cp))
-            {
-              ck = key;
-              cp = q;
-              cr = ri[u];
-            }
-          }
-        }
-      }
-      if (lane == 0)
-      {
-        r_key[nb][wv] = ck;
-        r_pos[nb][wv] = cp;
-        r_row[nb][wv] = cr;
-      }
-      if (tid == 0)
-      {
-        // GSL swaps positions k and pq.  phys[k] is not rewritten (it was read at the top of
-        // this step, possibly by a later wave); positions > k other than pq are read above.
-        phys[pq] = q0;
-        prow[k] = p;
-      }
-      __syncthreads ();
-    }
-
-    if (KEEP && s.LU)
-    {
-      double *out = s.LU + s.offA[cell];
-      for (int r = wv; r < n; r += NW)
-      {
-        const double *Mr = M + prow[r] * ld;
-        for (int c = lane; c < n; c += 64)
-          out[(size_t) r * n + c] = Mr[c];
-      }
-    }
-    if (singular)
-    {
-      if (tid == 0)
-        s.codes[cell] = 4;
-      __syncthreads ();
-      continue;
-    }
-
-    if (wv == 0)
-    {
-      double xr[3];
-      wave_backsub<3> (M, (size_t) ld, M + n, (size_t) ld, n, lane, xr, xs, prow);
-    }
-    __syncthreads ();
-
-    for (int i = tid; i < n; i += BS)
-    {
-      const double *Ai = A + (size_t) i * n;
-      double t = 0.0;
-#pragma unroll 8
-      for (int j = 0; j < n; j++)
-      {
-        const double pr = xs[j] * Ai[j];
-        t = t + pr;
-      }
-      const int f = residual_row_code (0.0 + t, b[i]);
-      if (f)
-        atomicMin (&s_fail, (i << 2) | f);
-      s.X[s.offV[cell] + i] = xs[i];
-    }
-    __syncthreads ();
-    if (tid == 0)
-      s.codes[cell] = (s_fail == 0x7fffffff) ? 0 : (s_fail & 3);
-    __syncthreads ();
-  }
-}
-
-// A value every lane of the wave holds (an LDS broadcast read), moved to a scalar register.
-__device__ __forceinline__ int uni (int v)
-{
-  return __builtin_amdgcn_readfirstlane (v);
-}
-
-// Row `row` of an n x n row-major matrix (row wave-uniform: a scalar pointer).
-__device__ __forceinline__ double *wrow (double *W, int n, int row)
-{
-  return W + (size_t) (unsigned) row * (unsigned) n;
-}
-
-// ============================================================================================
 // k_blocked_f64: right-looking blocked LU, NB-column panel in LDS, trailing matrix in a
 // per-workgroup HBM workspace (MALL / L2 resident between panels)
 // ============================================================================================
@@ -563,8 +392,8 @@ __device__ __forceinline__ double *wrow (double *W, int n, int row)
 //      updates in ascending kk with the product rounded first, the order of GSL's recursion.
 // Then the back substitution runs from LDS-staged chunks of U and the workgroup forms the
 // residual check.
-template <int BS, int NB, bool US, int MAXM, bool KEEP>
-__global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s, double *work, int64_t wstride)
+template <int BS, int NB, bool US, bool KEEP>
+__global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
 {
   extern __shared__ double smem[];
   constexpr int PP = NB + 2;           // panel row pitch (even: 16-byte aligned rows); column NB = b
@@ -585,8 +414,8 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s, double *wo
     const int n = s.n[cell];
     const double *A = s.A + s.offA[cell];
     const double *b = s.B + s.offV[cell];
-    double *W = (KEEP && s.LU) ? (s.LU + s.offA[cell]) : (work + (size_t) blockIdx.x * wstride);
-    double *wb = (KEEP && s.LU) ? (work + (size_t) blockIdx.x * wstride) : (W + (size_t) n * n);
+    double *W = s.F + s.offA[cell];     // factored in place in the factor store
+    double *wb = s.Y + s.offV[cell];
     double *P = smem;
     double *Us = P + (size_t) n * PP;
     double *xs = Us + (US ? (size_t) NB * n : 0);
@@ -849,105 +678,77 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s, double *wo
       __syncthreads ();
     }
 
-    if (singular)
-    {
-      if (tid == 0)
-        s.codes[cell] = 4;
-      __syncthreads ();
-      continue;
-    }
-
-    // ---- back substitution (dtrsv order) from LDS: U rows and y are staged in chunks of CR
-    // rows, double-buffered over the (now free) panel / U12 area; waves 1.. load the next chunk
-    // while wave 0 runs the chain of the current one ----
-    {
-      constexpr int CR = US ? 16 : 4;
-      const int pitch = (n + 2) & ~1;                  // U row (n) + y_i, 16-byte aligned rows
-      const int nch = (n + CR - 1) / CR;
-      double *cbuf = smem;                             // 2 x CR x pitch <= n x (PP + NB or 0)
-      auto stage = [&] (int c, int t0, int nt) {
-        const int hi = n - 1 - c * CR;
-        const int lo = max (0, hi - CR + 1);
-        double *dst = cbuf + (size_t) (c & 1) * CR * pitch;
-        const int cnt = (hi - lo + 1) * n;
-        for (int e = t0; e < cnt; e += nt)
-        {
-          const int r = e / n, j = e - r * n;
-          dst[r * pitch + j] = W[(size_t) (lo + r) * n + j];
-        }
-        for (int r = t0; r <= hi - lo; r += nt)
-          dst[r * pitch + n] = wb[lo + r];
-      };
-      stage (0, tid, BS);
-      __syncthreads ();
-      double xr[MAXM];
-#pragma unroll
-      for (int m = 0; m < MAXM; m++)
-        xr[m] = 0.0;
-      for (int c = 0; c < nch; c++)
-      {
-        if (wv > 0 && c + 1 < nch)
-          stage (c + 1, tid - 64, BS - 64);
-        if (wv == 0)
-        {
-          const int hi = n - 1 - c * CR;
-          const int lo = max (0, hi - CR + 1);
-          const double *buf = cbuf + (size_t) (c & 1) * CR * pitch;
-          for (int i = hi; i >= lo; i--)
-          {
-            const double *Ur = buf + (i - lo) * pitch;
-#pragma unroll
-            for (int m = 0; m < MAXM; m++)
-            {
-              const int j = lane + 64 * m;
-              if (j > i && j < n)
-                xs[j] = Ur[j] * xr[m];
-            }
-            const double tmp = chain_sub (Ur[n], xs, i + 1, n);
-            const double xi = tmp / Ur[i];
-#pragma unroll
-            for (int m = 0; m < MAXM; m++)
-              if (lane + 64 * m == i)
-                xr[m] = xi;
-          }
-        }
-        __syncthreads ();
-      }
-      if (wv == 0)
-      {
-#pragma unroll
-        for (int m = 0; m < MAXM; m++)
-          if (lane + 64 * m < n)
-            xs[lane + 64 * m] = xr[m];
-      }
-      __syncthreads ();
-    }
-
-    for (int i = tid; i < n; i += BS)
-    {
-      const double *Ai = A + (size_t) i * n;
-      double t = 0.0;
-#pragma unroll 8
-      for (int j = 0; j < n; j++)
-      {
-        const double pr = xs[j] * Ai[j];
-        t = t + pr;
-      }
-      const int f = residual_row_code (0.0 + t, b[i]);
-      if (f)
-        atomicMin (&s_fail, (i << 2) | f);
-      s.X[s.offV[cell] + i] = xs[i];
-    }
-    __syncthreads ();
     if (tid == 0)
-      s.codes[cell] = (s_fail == 0x7fffffff) ? 0 : (s_fail & 3);
+      s.codes[cell] = singular ? 4 : 0;
     __syncthreads ();
+  }
+}
+
+// ============================================================================================
+// k_bsub_f64: back substitution and residual check, one wave per cell
+// ============================================================================================
+// Reads the factor store and y the workgroup factor kernels wrote (GSL row order), runs the
+// dtrsv chain (wave_backsub) and the dgemv residual check, and writes x and the cell's code.
+// Cells the factor kernel found singular (code 4) are skipped: x is not written.  Many cells'
+// chains share each SIMD, so the serial chains of large matrices overlap instead of idling a
+// whole workgroup.
+template <int MAXM>
+__global__ __launch_bounds__ (256) void k_bsub_f64 (SolveArgs s)
+{
+  extern __shared__ double smem[];
+  constexpr int WPB = 4;
+  constexpr int XP = 64 * MAXM + 16;     // per-wave product / x staging (n + 7 readable)
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane (threadIdx.x >> 6);
+  double *xs = smem + wv * XP;
+  const int neff = list_len (s);
+  for (int item = blockIdx.x * WPB + wv; item < neff; item += gridDim.x * WPB)
+  {
+    const int cell = s.list[item];
+    if (s.codes[cell] == 4)
+      continue;
+    const int n = s.n[cell];
+    double xr[MAXM];
+    wave_backsub<MAXM> (s.F + s.offA[cell], (size_t) n, s.Y + s.offV[cell], 1, n, lane, xr, xs);
+
+    // residual check t = A x (gsl_blas_dgemv order) and codes 2 / 3 of the first failing row
+    const double *A = s.A + s.offA[cell];
+    const double *b = s.B + s.offV[cell];
+    int first = 0x7fffffff;
+#pragma unroll
+    for (int m = 0; m < MAXM; m++)
+    {
+      const int r = lane + 64 * m;
+      if (r < n)
+      {
+        const double *Ar = A + (size_t) r * n;
+        double t = 0.0;
+#pragma unroll 8
+        for (int j = 0; j < n; j++)
+        {
+          const double pr = Ar[j] * xs[j];
+          t = t + pr;
+        }
+        const int f = residual_row_code (0.0 + t, b[r]);
+        if (f)
+          first = min (first, (r << 2) | f);
+      }
+    }
+    first = wave_min_i32_dpp (first);
+#pragma unroll
+    for (int m = 0; m < MAXM; m++)
+      if (lane + 64 * m < n)
+        s.X[s.offV[cell] + lane + 64 * m] = xr[m];
+    if (lane == 0)
+      s.codes[cell] = (first == 0x7fffffff) ? 0 : (first & 3);
   }
 }
 
 // ============================================================================================
 // launchers
 // ============================================================================================
+static hipError_t launch_bsub_f64 (const SolveArgs &a, int nmax, hipStream_t st);
+
 size_t wave_f64_smem (int nc, int wpb, bool keep)
 {
   int per = 0;
@@ -966,24 +767,10 @@ size_t wave_f64_smem (int nc, int wpb, bool keep)
   return (size_t) wpb * ((per + 1) & ~1) * sizeof (double);
 }
 
-size_t lds_f64_smem (int n)
-{
-  const int ld = (n + 1) | 1;
-  return ((((size_t) n * ld + 1) & ~(size_t) 1) + (size_t) n) * sizeof (double) + 2 * (size_t) n * sizeof (int);
-}
 
-int lds_f64_limit ()
+int wgr_f64_limit ()
 {
-  static int cached = 0;
-  if (!cached)
-  {
-    const size_t cap = hipla_device_max_smem () - 2048;
-    int n = 1;
-    while (lds_f64_smem (n + 1) <= cap)
-      n++;
-    cached = n;
-  }
-  return cached;
+  return tile_f64_limit ();
 }
 
 // Blocked kernel variants: NB = 16 with U12 staged in LDS up to n = 512, NB = 8 reading U12 from
@@ -1033,57 +820,65 @@ hipError_t launch_wave_f64 (const SolveArgs &a, int nc, bool keep, int grid, hip
   }
 }
 
-hipError_t launch_lds_f64 (const SolveArgs &a, int nmax, bool keep, int grid, hipStream_t st)
+hipError_t launch_wgr_f64 (const SolveArgs &a, int nmax, bool keep, int grid, hipStream_t st)
 {
   if (a.nlist <= 0)
     return hipSuccess;
-  const size_t sm = lds_f64_smem (nmax);
-  grid = (grid > 0 && grid < a.nlist) ? grid : a.nlist;
-  // two matrices per CU when their LDS allows it (512 threads each), else one with 1024 threads
-  if (2 * (sm + 1024) <= hipla_device_max_smem ())
-  {
-    if (keep)
-      hipLaunchKernelGGL ((k_lds_f64<512, true>), dim3 (grid), dim3 (512), sm, st, a);
-    else
-      hipLaunchKernelGGL ((k_lds_f64<512, false>), dim3 (grid), dim3 (512), sm, st, a);
-  }
+  hipError_t e = launch_tile_f64 (a, nmax, keep, grid, st);
+  if (e != hipSuccess)
+    return e;
+  return launch_bsub_f64 (a, nmax, st);
+}
+
+template <int MAXM>
+static void launch_bsub_m (const SolveArgs &a, hipStream_t st)
+{
+  const int full = (a.nlist + 3) / 4;
+  const int grid = std::max (1, std::min (full, 4 * hipla_device_info ().num_cu));
+  const size_t sm = 4 * (64 * MAXM + 16) * sizeof (double);
+  hipLaunchKernelGGL ((k_bsub_f64<MAXM>), dim3 (grid), dim3 (256), sm, st, a);
+}
+
+// back substitution + residual check of the cells a workgroup factor kernel just factored
+static hipError_t launch_bsub_f64 (const SolveArgs &a, int nmax, hipStream_t st)
+{
+  if (nmax <= 128)
+    launch_bsub_m<2> (a, st);
+  else if (nmax <= 192)
+    launch_bsub_m<3> (a, st);
+  else if (nmax <= 256)
+    launch_bsub_m<4> (a, st);
+  else if (nmax <= 512)
+    launch_bsub_m<8> (a, st);
   else
-  {
-    if (keep)
-      hipLaunchKernelGGL ((k_lds_f64<1024, true>), dim3 (grid), dim3 (1024), sm, st, a);
-    else
-      hipLaunchKernelGGL ((k_lds_f64<1024, false>), dim3 (grid), dim3 (1024), sm, st, a);
-  }
+    launch_bsub_m<BLK_MAXM> (a, st);
   return hipGetLastError ();
 }
 
-template <int NB, bool US, int MAXM>
-static void launch_blocked_m (const SolveArgs &a, int grid, size_t sm, double *work, int64_t wstride, bool keep,
-                              hipStream_t st)
+template <int NB, bool US>
+static void launch_blocked_m (const SolveArgs &a, int grid, size_t sm, bool keep, hipStream_t st)
 {
   if (keep)
-    hipLaunchKernelGGL ((k_blocked_f64<BLK_BS, NB, US, MAXM, true>), dim3 (grid), dim3 (BLK_BS), sm, st, a, work,
-                        wstride);
+    hipLaunchKernelGGL ((k_blocked_f64<BLK_BS, NB, US, true>), dim3 (grid), dim3 (BLK_BS), sm, st, a);
   else
-    hipLaunchKernelGGL ((k_blocked_f64<BLK_BS, NB, US, MAXM, false>), dim3 (grid), dim3 (BLK_BS), sm, st, a, work,
-                        wstride);
+    hipLaunchKernelGGL ((k_blocked_f64<BLK_BS, NB, US, false>), dim3 (grid), dim3 (BLK_BS), sm, st, a);
 }
 
-hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, double *work, int64_t wstride, bool keep,
-                               hipStream_t st)
+hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, bool keep, hipStream_t st)
 {
   if (a.nlist <= 0)
     return hipSuccess;
-  if (nmax > 64 * BLK_MAXM)
+  if (nmax > 64 * BLK_MAXM || !a.F || !a.Y)
     return hipErrorInvalidValue;
   const size_t sm = blocked_f64_smem (nmax);
-  if (nmax <= 256)
-    launch_blocked_m<16, true, 4> (a, grid, sm, work, wstride, keep, st);
-  else if (nmax <= 512)
-    launch_blocked_m<16, true, 8> (a, grid, sm, work, wstride, keep, st);
+  if (nmax <= 512)
+    launch_blocked_m<16, true> (a, grid, sm, keep, st);
   else
-    launch_blocked_m<8, false, BLK_MAXM> (a, grid, sm, work, wstride, keep, st);
-  return hipGetLastError ();
+    launch_blocked_m<8, false> (a, grid, sm, keep, st);
+  hipError_t e = hipGetLastError ();
+  if (e != hipSuccess)
+    return e;
+  return launch_bsub_f64 (a, nmax, st);
 }
 
 // Name of the FP64 kernel instantiation a group of largest order nmax runs (as profilers print
@@ -1096,17 +891,17 @@ void fp64_kernel_symbol (int nmax, bool keep, char *buf, size_t len)
     const int nc = nmax <= 16 ? 16 : nmax <= 32 ? 32 : 64;
     snprintf (buf, len, "k_wave_f64<%d, %d, %s>", nc, nc == 64 ? 2 : 4, k);
   }
-  else if (nmax <= lds_f64_limit ())
+  else if (nmax <= wgr_f64_limit ())
   {
-    const size_t sm = lds_f64_smem (nmax);
-    snprintf (buf, len, "k_lds_f64<%d, %s>", 2 * (sm + 1024) <= hipla_device_max_smem () ? 512 : 1024, k);
+    char t[64];
+    tile_kernel_symbol (nmax, keep, t, sizeof t);
+    snprintf (buf, len, "%s + k_bsub_f64<%d>", t, nmax <= 128 ? 2 : 3);
   }
-  else if (nmax <= 256)
-    snprintf (buf, len, "k_blocked_f64<%d, 16, true, 4, %s>", BLK_BS, k);
   else if (nmax <= 512)
-    snprintf (buf, len, "k_blocked_f64<%d, 16, true, 8, %s>", BLK_BS, k);
+    snprintf (buf, len, "k_blocked_f64<%d, 16, true, %s> + k_bsub_f64<%d>", BLK_BS, k,
+              nmax <= 128 ? 2 : nmax <= 192 ? 3 : nmax <= 256 ? 4 : 8);
   else
-    snprintf (buf, len, "k_blocked_f64<%d, 8, false, %d, %s>", BLK_BS, BLK_MAXM, k);
+    snprintf (buf, len, "k_blocked_f64<%d, 8, false, %s> + k_bsub_f64<%d>", BLK_BS, k, BLK_MAXM);
 }
 
 // Best effort: AMD's runtime accepts dynamic LDS up to the device limit without the attribute;
@@ -1130,15 +925,14 @@ hipError_t setup_solve_f64_attrs (size_t max_smem)
   set_smem_attr (k_wave_f64<16, 4, true>, max_smem);
   set_smem_attr (k_wave_f64<32, 4, true>, max_smem);
   set_smem_attr (k_wave_f64<64, 2, true>, max_smem);
-  set_smem_attr (k_lds_f64<512, false>, max_smem);
-  set_smem_attr (k_lds_f64<512, true>, max_smem);
-  set_smem_attr (k_lds_f64<1024, false>, max_smem);
-  set_smem_attr (k_lds_f64<1024, true>, max_smem);
-  set_smem_attr (k_blocked_f64<BLK_BS, 16, true, 4, false>, max_smem);
-  set_smem_attr (k_blocked_f64<BLK_BS, 16, true, 4, true>, max_smem);
-  set_smem_attr (k_blocked_f64<BLK_BS, 16, true, 8, false>, max_smem);
-  set_smem_attr (k_blocked_f64<BLK_BS, 16, true, 8, true>, max_smem);
-  set_smem_attr (k_blocked_f64<BLK_BS, 8, false, BLK_MAXM, false>, max_smem);
-  set_smem_attr (k_blocked_f64<BLK_BS, 8, false, BLK_MAXM, true>, max_smem);
+  set_smem_attr (k_blocked_f64<BLK_BS, 16, true, false>, max_smem);
+  set_smem_attr (k_blocked_f64<BLK_BS, 16, true, true>, max_smem);
+  set_smem_attr (k_blocked_f64<BLK_BS, 8, false, false>, max_smem);
+  set_smem_attr (k_blocked_f64<BLK_BS, 8, false, true>, max_smem);
+  set_smem_attr (k_bsub_f64<2>, max_smem);
+  set_smem_attr (k_bsub_f64<3>, max_smem);
+  set_smem_attr (k_bsub_f64<4>, max_smem);
+  set_smem_attr (k_bsub_f64<8>, max_smem);
+  set_smem_attr (k_bsub_f64<BLK_MAXM>, max_smem);
   return hipSuccess;
 }
