@@ -57,10 +57,19 @@ def main(out=None):
 
     stats = list(csv.DictReader(open(os.path.join(out, "trace", "bench_kernel_stats.csv"))))
     trace = list(csv.DictReader(open(os.path.join(out, "trace", "bench_kernel_trace.csv"))))
-    dom_d = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in trace if r["Kernel_Name"].startswith("void " + dom)
-             or r["Kernel_Name"].startswith(dom)]
+    # the measured unit can be a kernel pair ("factor + back substitution", launched back to back)
+    parts = [x.strip() for x in dom.split(" + ")]
+
+    def is_part(name, part):
+        return name.startswith("void " + part + "(") or name.startswith(part + "(")
+
+    per_part = []
+    for part in parts:
+        d = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in trace if is_part(r["Kernel_Name"], part)]
+        per_part.append(d)
+    dom_d = per_part[0]
     os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
-    short = _short(dom)
+    short = _short(parts[0])
 
     lines = [f"# Kernel trace of the bench workload (uneven 10k-cell batch, FP64 size classes launched in isolation)",
              f"command: {CMD}", "",
@@ -68,11 +77,12 @@ def main(out=None):
     for r in stats[:14]:
         lines.append(f"{r['Name'][:78]:78s} {int(r['Calls']):6d} {float(r['AverageNs']) / 1e3:10.1f} "
                      f"{float(r['TotalDurationNs']) / 1e6:10.3f} {float(r['Percentage']):6.2f}")
-    avg_ms = sum(dom_d) / len(dom_d) / 1e6 if dom_d else float("nan")
+    part_avg = [sum(d) / len(d) / 1e6 if d else float("nan") for d in per_part]
+    avg_ms = sum(part_avg)
     lines += ["", f"dominant kernel: {dom}",
-              f"  dispatches in the trace: {len(dom_d)} (size classes {', '.join(roof['size_classes'])}, "
-              f"{roof['launches_per_step']} per repetition)",
-              f"  rocprofv3 average duration: {avg_ms:.4f} ms",
+              f"  dispatches in the trace: {', '.join(f'{p}: {len(d)}' for p, d in zip(parts, per_part))} "
+              f"(size classes {', '.join(roof['size_classes'])}, {roof['launches_per_step']} per repetition)",
+              f"  rocprofv3 average duration: {' + '.join(f'{a:.4f}' for a in part_avg)} = {avg_ms:.4f} ms",
               f"  bench.py HIP-event average: {roof['kernel_avg_ms']:.4f} ms per launch",
               f"  algorithmic bytes per launch: {roof['algorithmic_bytes_per_launch']}, flops per launch: "
               f"{roof['algorithmic_flops_per_launch']}",
@@ -90,32 +100,38 @@ def main(out=None):
                 return k, v
         raise RuntimeError("kernel not found")
 
-    _, f_dom = pick(fetch, lambda k: dom in k)
-    _, w_dom = pick(write, lambda k: dom in k)
     cname, f_cal = pick(fetch, lambda k: "MulFunctor" in k)
     _, w_cal = pick(write, lambda k: "MulFunctor" in k)
     kib = calib_bytes / 1024.0
     ff = max(f_cal) / kib
     fw = max(w_cal) / kib
-    fetch_b = sum(f_dom) / len(f_dom) * 1024.0 / ff
-    write_b = sum(w_dom) / len(w_dom) * 1024.0 / fw
+    f_kib = w_kib = 0.0
+    for part in parts:
+        _, fd = pick(fetch, lambda k: is_part(k, part))
+        _, wd = pick(write, lambda k: is_part(k, part))
+        f_kib += sum(fd) / len(fd)
+        w_kib += sum(wd) / len(wd)
+    f_dom = [f_kib]
+    w_dom = [w_kib]
+    fetch_b = f_kib * 1024.0 / ff
+    write_b = w_kib * 1024.0 / fw
     traffic = fetch_b + write_b
     tj = {"kernel": dom, "hbm_bytes_per_launch": round(traffic), "fetch_bytes_per_launch": round(fetch_b),
           "write_bytes_per_launch": round(write_b), "algorithmic_bytes_per_launch": roof["algorithmic_bytes_per_launch"],
-          "fetch_kib_per_dispatch_raw": round(sum(f_dom) / len(f_dom), 1),
-          "write_kib_per_dispatch_raw": round(sum(w_dom) / len(w_dom), 1),
+          "fetch_kib_per_launch_raw": round(f_kib, 1),
+          "write_kib_per_launch_raw": round(w_kib, 1),
           "calibration": {"kernel": cname[:90], "bytes_read_and_written": calib_bytes,
                           "fetch_factor": round(ff, 4), "write_factor": round(fw, 4)},
           "command": CMD}
     with open(os.path.join(ROOT, "profiles", "traffic.json"), "w") as f:
         json.dump(tj, f, indent=1)
         f.write("\n")
-    hl = [f"# HBM traffic of {dom} on the bench workload (per dispatch, {len(f_dom)} dispatches)",
+    hl = [f"# HBM traffic of {dom} on the bench workload (per launch)",
           f"command: {CMD}", "",
           f"calibration: {cname[:90]} reads and writes {calib_bytes} B; FETCH_SIZE reads {max(f_cal):.0f} KiB "
           f"(factor {ff:.4f}), WRITE_SIZE {max(w_cal):.0f} KiB (factor {fw:.4f})",
-          f"FETCH_SIZE {sum(f_dom) / len(f_dom):.1f} KiB per dispatch -> {fetch_b / 1e6:.2f} MB",
-          f"WRITE_SIZE {sum(w_dom) / len(w_dom):.1f} KiB per dispatch -> {write_b / 1e6:.2f} MB",
+          f"FETCH_SIZE {f_kib:.1f} KiB per launch -> {fetch_b / 1e6:.2f} MB",
+          f"WRITE_SIZE {w_kib:.1f} KiB per launch -> {write_b / 1e6:.2f} MB",
           f"HBM traffic per launch: {traffic / 1e6:.2f} MB against {roof['algorithmic_bytes_per_launch'] / 1e6:.2f} MB "
           f"algorithmic ({traffic / roof['algorithmic_bytes_per_launch']:.2f}x)"]
     with open(os.path.join(ROOT, "profiles", f"{short}_uneven10k_hbm.txt"), "w") as f:
