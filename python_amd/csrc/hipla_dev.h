@@ -319,6 +319,18 @@ __device__ __forceinline__ void combine_pivot_candidates (const long long *rkey,
   row = __builtin_amdgcn_readlane (r, src);
 }
 
+// v[slot] for a run-time slot, by bit masks: a select chain over a register array would be turned
+// back into a dynamically indexed array, which the compiler then keeps in scratch memory.
+template <int R>
+__device__ __forceinline__ double pick_slot (const double (&v)[R], int slot)
+{
+  unsigned long long acc = 0ull;
+#pragma unroll
+  for (int r = 0; r < R; r++)
+    acc |= (unsigned long long) __double_as_longlong (v[r]) & (0ull - (unsigned long long) (slot == r));
+  return __longlong_as_double ((long long) acc);
+}
+
 // Uniform read of a double held by lane `src`.
 __device__ __forceinline__ double rl_f64 (double v, int src)
 {

@@ -399,8 +399,10 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
   constexpr int PP = NB + 2;           // panel row pitch (even: 16-byte aligned rows); column NB = b
   constexpr int NW = BS / 64;
   constexpr int RB = 8;                // rows per trailing-update tile
-  __shared__ long long r_key[NW];
-  __shared__ int r_pos[NW], r_row[NW];
+  constexpr int PR = US ? 1 : 2;       // panel rows per thread (n <= 512: one, else two)
+  __shared__ long long c_key[2][NW];   // per-wave pivot candidates, by step parity
+  __shared__ int c_pos[2][NW], c_row[2][NW];
+  __shared__ double c_val[2][NW][NB + 2];
   __shared__ int s_ipiv[NB];
   __shared__ int mv_dst[NB], mv_src[NB];
   __shared__ int s_nmv, s_fail;
@@ -451,100 +453,113 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
       const int nbk = min (NB, n - k0);
       const int rows = n - k0;
       const int k1 = k0 + nbk;
-      for (int r = tid; r < rows; r += BS)
+      // ---- 1. panel factorization in registers: thread t holds panel rows t + BS pr ----
+      // Each step publishes, per wave, the best candidate (GSL key, position) together with
+      // that row's values, so after one barrier every thread can eliminate against the winner.
+      double v[PR][NB + 1];            // panel columns k0 .. k0+NB-1, then b
+      int mp[PR];                      // position of each held row (relative to k0)
+#pragma unroll
+      for (int pr = 0; pr < PR; pr++)
       {
-        const double *src = W + (size_t) (k0 + r) * n + k0;
-        double *dst = P + (size_t) r * PP;
-        for (int c = 0; c < nbk; c++)
-          dst[c] = src[c];
-        dst[NB] = wb[k0 + r];
-        pos[r] = r;
-        phys[r] = r;
+        const int r = tid + BS * pr;
+        mp[pr] = r < rows ? r : 0x7fffffff;
+        const double *src = W + (size_t) (k0 + min (r, rows - 1)) * n + k0;
+#pragma unroll
+        for (int c = 0; c < NB; c++)
+          v[pr][c] = (r < rows && c < nbk) ? src[c] : 0.0;
+        v[pr][NB] = r < rows ? wb[k0 + r] : 0.0;
       }
       if (tid == 0)
         s_nmv = 0;
-      __syncthreads ();
-
-      // ---- 1. panel factorization ----
-      for (int kk = 0; kk < nbk; kk++)
+#pragma clang loop unroll(full)
+      for (int kk = 0; kk < NB; kk++)
       {
-        long long key = -1;
-        int bp = 0x7fffffff, br = 0;
-        for (int r = tid; r < rows; r += BS)
+        if (kk < nbk)
         {
-          const int q = pos[r];
-          if (q >= kk)
-          {
-            const long long k = pivot_key (P[(size_t) r * PP + kk]);
-            if (k > key || (k == key && q < bp))
+          const int buf = kk & 1;
+          long long key = -1;
+          int bp = 0x7fffffff, bs = 0;
+#pragma unroll
+          for (int pr = 0; pr < PR; pr++)
+            if (mp[pr] >= kk && mp[pr] < rows)
             {
-              key = k;
-              bp = q;
-              br = r;
+              const long long q = pivot_key (v[pr][kk]);
+              if (q > key || (q == key && mp[pr] < bp))
+              {
+                key = q;
+                bp = mp[pr];
+                bs = pr;
+              }
+            }
+          const int wl = wave_pivot_lane (key >= 0, key, bp);
+          if (lane == (wl >= 0 ? wl : 0))
+          {
+            c_key[buf][wv] = wl >= 0 ? key : -1;
+            c_pos[buf][wv] = wl >= 0 ? bp : 0x7fffffff;
+            c_row[buf][wv] = tid + BS * bs;
+#pragma unroll
+            for (int c = kk; c <= NB; c++)
+            {
+              double col[PR];
+#pragma unroll
+              for (int pr = 0; pr < PR; pr++)
+                col[pr] = v[pr][c];
+              c_val[buf][wv][c] = pick_slot<PR> (col, bs);
             }
           }
-        }
-        const int wl = wave_pivot_lane (key >= 0, key, bp);
-        if (lane == 0)
-        {
-          r_key[wv] = wl >= 0 ? (long long) rl_u64 ((unsigned long long) key, wl) : -1;
-          r_pos[wv] = wl >= 0 ? __builtin_amdgcn_readlane (bp, wl) : 0x7fffffff;
-          r_row[wv] = wl >= 0 ? __builtin_amdgcn_readlane (br, wl) : 0;
-        }
-        __syncthreads ();
-        int pq, p;
-        combine_pivot_candidates (r_key, r_pos, r_row, NW, lane, pq, p);
-        const double *Pp = P + (size_t) p * PP;
-        const double piv = Pp[kk];
-        singular |= (piv == 0.0);
-        const bool recip = fabs (piv) >= DBL_MIN;
-        const double sc = 1.0 / piv;
-        // every row still below the pivot: the row at position kk (moving to pq) included
-        // (the whole row and the pivot row are read into registers, updated, written back)
-        for (int r = tid; r < rows; r += BS)
-        {
-          if (r == p || pos[r] < kk)
-            continue;
-          double *Pr = P + (size_t) r * PP;
-          const double ark = Pr[kk];
-          double v[PP];
+          __syncthreads ();
+          int pq, p;
+          combine_pivot_candidates (c_key[buf], c_pos[buf], c_row[buf], NW, lane, pq, p);
+          const double *u = c_val[buf][(p & (BS - 1)) >> 6];
+          const double piv = u[kk];
+          singular |= (piv == 0.0);
+          const bool recip = fabs (piv) >= DBL_MIN;
+          const double sc = 1.0 / piv;
 #pragma unroll
-          for (int c = 0; c < PP; c += 2)
+          for (int pr = 0; pr < PR; pr++)
           {
-            const double2 x = *reinterpret_cast<const double2 *> (Pr + c);
-            v[c] = x.x;
-            v[c + 1] = x.y;
-          }
-          double l;
-          if (recip)
-            l = ark * sc;
-          else
-            l = ark / piv;
+            const int r = tid + BS * pr;
+            // every row still below the pivot, the row at position kk (moving to pq) included
+            if (r != p && mp[pr] >= kk && mp[pr] < rows)
+            {
+              double l;
+              if (recip)
+                l = v[pr][kk] * sc;
+              else
+                l = v[pr][kk] / piv;
+              v[pr][kk] = l;
 #pragma unroll
-          for (int c = 0; c < PP; c++)
-          {
-            if (c == NB || (c > kk && c < nbk))
-              v[c] = v[c] - l * Pp[c];
-            else if (c == kk)
-              v[c] = l;
+              for (int c = kk + 1; c < NB; c++)
+                if (c < nbk)
+                  v[pr][c] = v[pr][c] - l * u[c];
+              v[pr][NB] = v[pr][NB] - l * u[NB];
+            }
+            // GSL swaps the rows at positions kk and pq
+            if (r == p)
+              mp[pr] = kk;
+            else if (mp[pr] == kk)
+              mp[pr] = pq;
           }
-#pragma unroll
-          for (int c = 0; c < PP; c += 2)
-            *reinterpret_cast<double2 *> (Pr + c) = make_double2 (v[c], v[c + 1]);
+          if (tid == 0)
+            s_ipiv[kk] = pq;
         }
-        if (tid == 0)
-        {
-          // GSL swaps the rows at positions kk and pq.  The elimination above reads pos[] of
-          // other rows only; the row moved here is eliminated whether it reads kk or pq.
-          const int q0 = phys[kk];
-          phys[pq] = q0;
-          pos[q0] = pq;
-          phys[kk] = p;
-          pos[p] = kk;
-          s_ipiv[kk] = pq;
-        }
-        __syncthreads ();
       }
+      // factored panel rows to LDS and the position maps, for the U12 / A22 phases
+#pragma unroll
+      for (int pr = 0; pr < PR; pr++)
+      {
+        const int r = tid + BS * pr;
+        if (r < rows)
+        {
+          double *dst = P + (size_t) r * PP;
+#pragma unroll
+          for (int c = 0; c <= NB; c++)
+            dst[c] = v[pr][c];
+          pos[r] = mp[pr];
+          phys[mp[pr]] = r;
+        }
+      }
+      __syncthreads ();
 
       // ---- 2a. panel back in position order; rows moved below the panel ----
       for (int q = tid; q < rows; q += BS)

@@ -22,18 +22,6 @@
 #include "hipla_dev.h"
 #include "hipla_kernels.h"
 
-// v[slot] for a run-time slot, by bit masks: a select chain over a register array would be turned
-// back into a dynamically indexed array, which the compiler then keeps in scratch memory.
-template <int R>
-__device__ __forceinline__ double pick_slot (const double (&v)[R], int slot)
-{
-  unsigned long long acc = 0ull;
-#pragma unroll
-  for (int r = 0; r < R; r++)
-    acc |= (unsigned long long) __double_as_longlong (v[r]) & (0ull - (unsigned long long) (slot == r));
-  return __longlong_as_double ((long long) acc);
-}
-
 template <int R, int CPW, int NW, bool KEEP>
 __global__ __launch_bounds__ (64 * NW, (NW == 8) ? 4 : 4) void k_tile_f64 (SolveArgs s)
 {
