@@ -82,113 +82,95 @@ __device__ __forceinline__ double sub_oct (double t, const Oct &o)
   return t;
 }
 
-// Only the elements g+u with lo <= g+u < hi (wave-uniform bounds).
-__device__ __forceinline__ double sub_oct_masked (double t, const Oct &o, int g, int lo, int hi)
-{
-  const double v[8] = { o.a.x, o.a.y, o.b.x, o.b.y, o.c.x, o.c.y, o.d.x, o.d.y };
-#pragma unroll
-  for (int u = 0; u < 8; u++)
-    if (g + u >= lo && g + u < hi)
-      t = t - v[u];
-  return t;
-}
-
-// tmp - p[j] - p[j+1] - ... - p[n-1], subtracted in ascending order (a dependent chain); p in
-// LDS, 16-byte aligned, and readable up to p[n+6].  The products are read in groups of eight,
-// the next group's reads in flight while the current group is subtracted (two groups per turn
-// of the loop, so the two stay in separate registers).
+// tmp - p[j] - p[j+1] - ... - p[n-1], subtracted in ascending order (a dependent chain).  p is
+// in LDS, 16-byte aligned, and holds +0.0 at every index below j (down to j rounded to even) and
+// from n up to n rounded up to a multiple of eight: the chain runs over whole groups of eight,
+// and subtracting +0.0 leaves every value unchanged (IEEE: t - (+0) == t, -0 and NaN included),
+// so the padding changes nothing.  The next group's reads are in flight while the current group
+// is subtracted.
 __device__ __forceinline__ double chain_sub (double tmp, const double *p, int j, int n)
 {
-  if (j >= n)
-    return tmp;
   int g = j & ~1;
+  if (g >= n)
+    return tmp;
   Oct c = load_oct (p + g);
-  if (g + 8 >= n || g < j)
+  for (; g + 8 < n; g += 8)
   {
-    // the first group is partial (or the only one)
-    const Oct d = (g + 8 < n) ? load_oct (p + g + 8) : c;
-    __builtin_amdgcn_sched_barrier (0);
-    tmp = sub_oct_masked (tmp, c, g, j, n);
-    if (g + 8 >= n)
-      return tmp;
-    c = d;
-    g += 8;
-  }
-  // here c = p[g .. g+7] with g >= j
-  while (true)
-  {
-    if (g + 8 >= n)
-      return sub_oct_masked (tmp, c, g, g, n);
     const Oct d = load_oct (p + g + 8);
     __builtin_amdgcn_sched_barrier (0);
     tmp = sub_oct (tmp, c);
-    g += 8;
-    if (g + 8 >= n)
-      return sub_oct_masked (tmp, d, g, g, n);
-    c = load_oct (p + g + 8);
-    __builtin_amdgcn_sched_barrier (0);
-    tmp = sub_oct (tmp, d);
-    g += 8;
+    c = d;
   }
+  return sub_oct (tmp, c);
 }
 
-// Back substitution U x = y in dtrsv order, run by one wave.  Row i of U is at U + r*ldu
-// (columns i..n-1 at their own column index) and y_i at y[r*ys], with r = rmap[i] when a row map
-// is given, else r = i.  Lane l keeps x_{l + 64 m} in xr[m].  For row i every lane forms its
-// products U_ij x_j in parallel and stages them in xs (n doubles of LDS); the wave then subtracts
-// them in ascending j from broadcast reads -- the chain of dependent FP64 subtractions of GSL's
-// dtrsv -- and divides by U_ii.  Row i-1 is fetched while row i is processed.  xs must be
-// 16-byte aligned.  On return xs holds x and xr is filled.
+// Back substitution U x = y in dtrsv order, run by one wave.  Row i of U is at U + i*ldu
+// (columns i..n-1 at their own column index) and y_i at y[i*ys].  Lane l keeps x_{l + 64 m} in
+// xr[m].  For row i every lane forms its products U_ij x_j in parallel and stages them in xs (LDS,
+// 16-byte aligned, zeros elsewhere); the wave then subtracts them in ascending j (the
+// chain of dependent FP64 subtractions of GSL's dtrsv, cblas/source_trsv_r.h:35-52) and divides
+// by U_ii.  Four rows are in flight: the row loaded into a slot is used four rows later, so the
+// short chains near the bottom of U do not wait on memory.  xs needs n + 8 doubles.  On return
+// xs holds x.
 template <int MAXM>
 __device__ __forceinline__ void wave_backsub (const double *U, size_t ldu, const double *y, size_t ys, int n, int lane,
-                                              double (&xr)[MAXM], double *xs, const int *rmap = nullptr)
+                                              double (&xr)[MAXM], double *xs)
 {
+  constexpr int D = 4;
 #pragma unroll
   for (int m = 0; m < MAXM; m++)
     xr[m] = 0.0;
-  int ri = rmap ? __builtin_amdgcn_readfirstlane (rmap[n - 1]) : n - 1;
-  double cu[MAXM], cy, cd;
+  if (lane < 8)
+    xs[n + lane] = 0.0;          // padding of the last group (see chain_sub)
+  double cu[D][MAXM], cy[D], cd[D];
 #pragma unroll
-  for (int m = 0; m < MAXM; m++)
+  for (int sl = 0; sl < D; sl++)
   {
-    const int j = lane + 64 * m;
-    cu[m] = j < n ? U[(size_t) ri * ldu + j] : 0.0;
-  }
-  cy = y[(size_t) ri * ys];
-  cd = U[(size_t) ri * ldu + (n - 1)];
-  for (int i = n - 1; i >= 0; i--)
-  {
-    double nu[MAXM], ny = 0.0, nd = 1.0;
-    if (i > 0)
-    {
-      const int rn = rmap ? __builtin_amdgcn_readfirstlane (rmap[i - 1]) : i - 1;
-#pragma unroll
-      for (int m = 0; m < MAXM; m++)
-      {
-        const int j = lane + 64 * m;
-        nu[m] = j < n ? U[(size_t) rn * ldu + j] : 0.0;
-      }
-      ny = y[(size_t) rn * ys];
-      nd = U[(size_t) rn * ldu + (i - 1)];
-    }
+    const int r = max (n - 1 - sl, 0);
 #pragma unroll
     for (int m = 0; m < MAXM; m++)
     {
       const int j = lane + 64 * m;
-      if (j > i && j < n)
-        xs[j] = cu[m] * xr[m];
+      cu[sl][m] = j < n ? U[(size_t) r * ldu + j] : 0.0;
     }
-    const double tmp = chain_sub (cy, xs, i + 1, n);
-    const double xi = tmp / cd;
+    cy[sl] = y[(size_t) r * ys];
+    cd[sl] = U[(size_t) r * ldu + r];
+  }
+  for (int i0 = n - 1; i0 >= 0; i0 -= D)
+  {
 #pragma unroll
-    for (int m = 0; m < MAXM; m++)
-      if (lane + 64 * m == i)
-        xr[m] = xi;
+    for (int sl = 0; sl < D; sl++)
+    {
+      const int i = i0 - sl;
+      if (i >= 0)
+      {
 #pragma unroll
-    for (int m = 0; m < MAXM; m++)
-      cu[m] = nu[m];
-    cy = ny;
-    cd = nd;
+        for (int m = 0; m < MAXM; m++)
+        {
+          const int j = lane + 64 * m;
+          if (j < n)
+            xs[j] = j > i ? cu[sl][m] * xr[m] : 0.0;
+        }
+        const double tmp = chain_sub (cy[sl], xs, i + 1, n);
+        const double xi = tmp / cd[sl];
+#pragma unroll
+        for (int m = 0; m < MAXM; m++)
+          if (lane + 64 * m == i)
+            xr[m] = xi;
+        const int r = i - D;       // refill this slot with the row it serves next
+        if (r >= 0)
+        {
+#pragma unroll
+          for (int m = 0; m < MAXM; m++)
+          {
+            const int j = lane + 64 * m;
+            cu[sl][m] = j < n ? U[(size_t) r * ldu + j] : 0.0;
+          }
+          cy[sl] = y[(size_t) r * ys];
+          cd[sl] = U[(size_t) r * ldu + r];
+        }
+      }
+    }
   }
 #pragma unroll
   for (int m = 0; m < MAXM; m++)
@@ -708,7 +690,7 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
 // chains share each SIMD, so the serial chains of large matrices overlap instead of idling a
 // whole workgroup.
 template <int MAXM>
-__global__ __launch_bounds__ (256) void k_bsub_f64 (SolveArgs s)
+__global__ __launch_bounds__ (256, MAXM <= 4 ? 4 : 1) void k_bsub_f64 (SolveArgs s)
 {
   extern __shared__ double smem[];
   constexpr int WPB = 4;
