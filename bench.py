@@ -97,6 +97,155 @@ def measure_roofline(plan, A, B, reps=10):
     return roof, classes
 
 
+def _timed_ms(fn, steps, warmup):
+    """Average ms per call of fn() between HIP events on torch's current stream."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def _emit(out):
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_c3(args):
+    """Wide-range rates (1e-20..1e4), the same 10,000-cell size mix, seed 3: the mixed solver with
+    every size class on the FP32 path (no tuning), then autotuned; per-cell error against the
+    oracle, refinement-step histogram, fallback rate."""
+    import torch
+    import oracle
+    import python_amd as pa
+    from python_amd.generators import uneven
+    batch = uneven(seed=3, wide=True, scale=0.1 if args.quick else 1.0)
+    A, B = pa.to_device(batch)
+    oracle.build()
+    Xo, co = oracle.solve_batch(batch.n, batch.offA, batch.offV, batch.A, batch.B, nthreads=_cores())
+    out = {"metric": "solves/s, wide-range rate matrices (1e-20..1e4), mixed precision", "unit": "solves/s",
+           "n_gpus": 1, "data": "synthetic (seeded ionization-balance generator, wide rates, seed 3)",
+           "config": {"workload": f"uneven_wide_10k: {batch.ncells} cells, n in {{12..256}}"}, "modes": {}}
+    for mode, tune in (("fp32_refined_untuned", False), ("autotuned", True)):
+        plan = pa.Plan.for_batch(batch, mode="mixed", graph=True)
+        X = torch.zeros_like(B)
+        codes = torch.empty(batch.ncells, dtype=torch.int32, device=B.device)
+        st = torch.zeros(batch.ncells * 24, dtype=torch.uint8, device=B.device)   # hipla_cell_stats records
+        if tune:
+            plan.autotune(A, B, X, codes)
+        ms = _timed_ms(lambda: plan.solve(A, B, X, codes, st), args.steps, args.warmup)
+        Xh, ch = X.cpu().numpy(), codes.cpu().numpy()
+        stats = pa.stats_to_numpy(st)
+        rel = np.zeros(batch.ncells)
+        for c in np.where(co != 4)[0]:
+            o, n = batch.offV[c], batch.n[c]
+            rel[c] = np.abs(Xh[o:o + n] - Xo[o:o + n]).max() / max(np.abs(Xo[o:o + n]).max(), 1e-300)
+        fb = stats["fallback"]
+        out["modes"][mode] = {
+            "solves_per_s": round(batch.ncells / (ms * 1e-3), 1), "ms_per_step": round(ms, 4),
+            "max_rel_err_vs_fp64_oracle": float(rel.max()), "cells_within_1e-8": int((rel <= 1e-8).sum()),
+            "codes_equal_oracle": bool(np.array_equal(ch, co)),
+            "fallback_rate": round(float((fb == 1).sum()) / batch.ncells, 4),
+            "fp64_direct_cells": int((fb == 2).sum()),
+            "refine_steps_hist": np.bincount(stats["refine_steps"][fb == 0], minlength=1).tolist(),
+            "per_size": {int(n): {"cells": int((batch.n == n).sum()),
+                                  "fallback": int(((fb == 1) & (batch.n == n)).sum()),
+                                  "max_rel_err": float(rel[batch.n == n].max())} for n in np.unique(batch.n)}}
+        print(f"[bench c3] {mode}: {ms:.3f} ms", file=sys.stderr, flush=True)
+        plan.close()
+    out["value"] = out["modes"]["autotuned"]["solves_per_s"]
+    out["singular_cells_code4"] = int((co == 4).sum())
+    return _emit(out)
+
+
+def bench_c4(args):
+    """Macro-atom inversion: 2,000 cells N = I - Q, n in {256, 512, 1024} (thirds), seed 4; FP64
+    inverse in GSL's LU order; relative Frobenius error against the oracle on a sample of cells."""
+    import torch
+    import oracle
+    import python_amd as pa
+    from python_amd.generators import macro_inverse
+    t0 = time.time()
+    batch = macro_inverse(ncells=200 if args.quick else 2000)
+    print(f"[bench c4] generated {batch.ncells} cells in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    A, _ = pa.to_device(batch)
+    plan = pa.Plan.for_batch(batch, mode="fp64", graph=False)
+    Inv = torch.empty_like(A)
+    codes = torch.empty(batch.ncells, dtype=torch.int32, device=A.device)
+    steps = max(1, min(args.steps, 3))
+    ms = _timed_ms(lambda: plan.invert(A, Inv, codes), steps, 1)
+    flops = float(sum(2.0 * n ** 3 for n in batch.n))
+    byts = float(sum(16.0 * n * n for n in batch.n))
+    Ih, ch = Inv.cpu().numpy(), codes.cpu().numpy()
+    oracle.build()
+    errs = {}
+    for nsz in np.unique(batch.n):
+        for c in np.where(batch.n == nsz)[0][:2]:
+            a, _ = batch.cell(c)
+            code, ref = oracle.invert_matrix(a)
+            got = Ih[batch.offA[c]:batch.offA[c] + nsz * nsz].reshape(nsz, nsz)
+            errs.setdefault(int(nsz), []).append(float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
+    tf = flops / (ms * 1e-3) / 1e12
+    return _emit({
+        "metric": "inverses/s, macro-atom matrices N = I - Q, n in {256,512,1024}", "unit": "inverses/s",
+        "value": round(batch.ncells / (ms * 1e-3), 2), "ms_per_step": round(ms, 3), "steps": steps, "n_gpus": 1,
+        "dtype": "f64", "data": "synthetic (seeded substochastic Q, seed 4)",
+        "config": {"workload": f"macro_inverse: {batch.ncells} cells, n in {{256,512,1024}} thirds"},
+        "codes_nonzero": int((ch != 0).sum()),
+        "rel_frobenius_err_sample": errs, "max_rel_frobenius_err_sample": max(max(v) for v in errs.values()),
+        "roofline": {"bound": "mfma", "achieved": round(tf, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(tf / FP64_PEAK_TFLOPS, 5), "hbm_achieved_gbs": round(byts / (ms * 1e-3) / 1e9, 2)}})
+
+
+def bench_c5(args):
+    """T_e gradient: 10,000 cells of n = 64 whose rates depend on T_e, seed 5.  A step is the torch
+    assembly of A(T_e), the batched FP64 solve (factors kept), the adjoint A^T lambda = dL/dx and
+    the chain to dL/dT_e; the gradient is checked against the FP64 oracle on a sample."""
+    import torch
+    import oracle
+    from python_amd.adjoint import IonTe, loss_and_grad
+    from python_amd.generators import ion_matrices_te, te_assemble
+    rng = np.random.default_rng(5)
+    ncells = 1000 if args.quick else 10000
+    d = ion_matrices_te(rng, ncells, 64)
+    w = rng.random((ncells, 64))
+    model = IonTe(d)
+    plan = model.plan()
+    T = torch.from_numpy(d["te"]).cuda()
+    wt = torch.from_numpy(w).cuda()
+    res = {}
+
+    def step():
+        res["out"] = loss_and_grad(model, plan, T, wt)
+
+    ms = _timed_ms(step, args.steps, args.warmup)
+    L, g, codes = res["out"]
+    g = g.cpu().numpy()
+    oracle.build()
+    A, dA, b = te_assemble(64, d["blocks"], d["params"], d["te"])
+    worst = 0.0
+    for c in range(0, ncells, max(1, ncells // 200)):
+        LU, perm, _, _ = oracle.lu_decomp(A[c])
+        _, x = oracle.lu_solve(LU, perm, b[c])
+        _, lam = oracle.lu_solve_transpose(LU, perm, w[c])
+        gref = -lam @ (dA[c] @ x)
+        mag = np.abs(lam) @ (np.abs(dA[c]) @ np.abs(x))
+        worst = max(worst, abs(gref - g[c]) / (1e-8 * abs(gref) + 1e-10 * mag))
+    return _emit({
+        "metric": "gradients/s, dL/dT_e through the adjoint, n = 64", "unit": "cells/s",
+        "value": round(ncells / (ms * 1e-3), 1), "ms_per_step": round(ms, 4), "n_gpus": 1, "dtype": "f64",
+        "data": "synthetic (T_e-dependent ionization matrices, seed 5)",
+        "config": {"workload": f"te_gradient: {ncells} cells, n = 64, loss = sum_i w_i x_i"},
+        "gradient_error_over_bound_max": round(worst, 6),
+        "gradient_bound": "1e-8 |g| + 1e-10 sum|lambda||dA||x| against the oracle on 200 sampled cells"})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +256,9 @@ def main():
     ap.add_argument("--quick", action="store_true", help="1/10 of the batch (development only)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="plan set-up, graph timing and the dominant-kernel measurement only (profiler runs)")
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5"],
+                    help="c2: the headline batch (default); c3: wide-range rates, refinement statistics; "
+                         "c4: macro-atom inversion; c5: T_e gradient (forward solve + adjoint)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs on a single MI355X (the cell batch would shard "
@@ -117,6 +269,9 @@ def main():
     import torch
     import python_amd as pa
     from python_amd.generators import uneven
+
+    if args.config != "c2":
+        return {"c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.config](args)
 
     t_gen = time.time()
     batch = uneven(scale=0.1 if args.quick else 1.0)
