@@ -286,11 +286,34 @@ __device__ __forceinline__ int wave_min_i32_dpp (int v)
   return r;
 }
 
+// Wave-uniform maximum of an unsigned 32-bit value (all 64 lanes active): within each row of 16
+// lanes by quad and mirror exchanges, then across rows with the row_bcast15 / row_bcast31
+// broadcasts, which leave the total in lane 63.
+__device__ __forceinline__ unsigned wave_max_u32 (unsigned v)
+{
+  v = max (v, (unsigned) __builtin_amdgcn_update_dpp (0, (int) v, 0xB1, 0xF, 0xF, false));
+  v = max (v, (unsigned) __builtin_amdgcn_update_dpp (0, (int) v, 0x4E, 0xF, 0xF, false));
+  v = max (v, (unsigned) __builtin_amdgcn_update_dpp (0, (int) v, 0x141, 0xF, 0xF, false));
+  v = max (v, (unsigned) __builtin_amdgcn_update_dpp (0, (int) v, 0x140, 0xF, 0xF, false));
+  v = max (v, (unsigned) __builtin_amdgcn_update_dpp ((int) v, (int) v, 0x142, 0xA, 0xF, false));
+  v = max (v, (unsigned) __builtin_amdgcn_update_dpp ((int) v, (int) v, 0x143, 0xC, 0xF, false));
+  return (unsigned) __builtin_amdgcn_readlane ((int) v, 63);
+}
+
 // GSL pivot choice within a wave: among lanes with `alive`, the largest pivot_key, ties to the
-// smallest `pos`.  Returns the winning lane (uniform), or -1 when no lane is alive.
+// smallest `pos`.  Returns the winning lane (uniform), or -1 when no lane is alive.  The high 32
+// bits of the key are reduced first: when a single lane holds their maximum, its key is the
+// largest outright; otherwise the full keys (and positions, for ties) decide.
 __device__ __forceinline__ int wave_pivot_lane (bool alive, long long key, int pos)
 {
   const unsigned long long k1 = alive ? (unsigned long long) key + 1ull : 0ull;
+  const unsigned hi = (unsigned) (k1 >> 32);
+  const unsigned hm = wave_max_u32 (hi);
+  const unsigned long long c0 = __ballot (k1 != 0ull && hi == hm);
+  if (c0 == 0ull)
+    return -1;
+  if (__popcll (c0) == 1)
+    return __ffsll ((long long) c0) - 1;
   const unsigned long long mk = wave_max_u64_dpp (k1);
   if (mk == 0ull)
     return -1;
