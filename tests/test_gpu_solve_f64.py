@@ -134,3 +134,26 @@ def test_per_cell_and_graph_replays_match(hip, oracle_lib):
             assert _eq(got[0], ref[0]) and np.array_equal(got[1], ref[1]), (per_cell, graph, rep)
             _ = (A * 1.0).sum().item()          # eager work between replays
         plan.close()
+
+
+def test_large_orders_equal_oracle(hip, oracle_lib):
+    """Orders past the 512 boundary of the blocked kernel (8-column panels, two panel rows per
+    thread, U12 read from the factor store): values equal the oracle."""
+    from python_amd.generators import pack
+    rng = np.random.default_rng(21)
+    mats, rhs = [], []
+    for n in (513, 600, 1024):
+        A = rng.standard_normal((n, n))
+        A[np.arange(n), np.arange(n)] += 0.1
+        mats.append(A)
+        rhs.append(rng.standard_normal(n))
+    batch = pack(mats, rhs)
+    X, codes, _ = hip.solve_batch(batch, mode="fp64")
+    for c in range(batch.ncells):
+        a, b = batch.cell(c)
+        code, xo = oracle_lib.solve_matrix(a, b)
+        n = a.shape[0]
+        print("n", n, "code", codes[c], code, flush=True)
+        assert codes[c] == code
+        if code != 4:
+            assert _eq(X[batch.offV[c]:batch.offV[c] + n], xo), n
