@@ -184,22 +184,67 @@ __device__ __forceinline__ void wave_backsub (const double *U, size_t ldu, const
 // ============================================================================================
 // k_wave_f64: one wave per matrix
 // ============================================================================================
-// Per wave: the lane whose row is chosen as pivot at step k keeps that row's remaining columns
-// (U(k, k..n-1)) and y_k in its registers from then on; the other lanes read the row from a small
-// LDS broadcast buffer.  A is loaded coalesced through a staging tile of SR rows; LDS per wave is
-// the tile, the broadcast row and x (plus the multipliers when the factors are kept).
+// Lane r holds row r of [A | b] in registers; columns are shifted left as they are eliminated, so
+// a[0] is always the current column and a[t] the column t to its right.  Registers past a row's
+// live columns hold +0 (loaded so and kept so by the shifted update), and so does the broadcast
+// row past its live entries: the update and the back substitution then run over whole groups of
+// eight columns under wave-uniform guards, the padded terms being exact no-ops (x - (+0) == x,
+// +0 - l * (+0) == +0).  The lane whose row is chosen as pivot at step k keeps U(k, k..n-1) and
+// y_k in its registers from then on; the other lanes read the row from a small LDS broadcast
+// buffer.  A is loaded (and re-read for the residual check) with coalesced flat loads through a
+// staging tile of SR rows.  LDS per wave: the tile, the broadcast row, x with its padding (plus
+// the multipliers and pivots when the factors are kept).
 template <int NC>
 struct WaveF64Layout
 {
-  static constexpr int SR = NC < 16 ? NC : 16;   // rows per staging pass of the A load
-  static constexpr int SP = NC | 1;              // staging pitch (odd: fewer bank conflicts)
+  static constexpr int SR = NC < 16 ? NC : 16;   // rows per staging pass of A
+  static constexpr int SP = NC | 1;              // staging pitch (odd: no bank conflicts)
   static constexpr int BR = NC + 2;              // broadcast row: [t] = U(k, k+t), [NC] = y_k
+  static constexpr int XS = NC + 8;              // x, then +0 up to the next group of eight
   static constexpr int LDL = NC + 1;             // KEEP: multipliers of one row
   static constexpr int base (bool keep)
   {
-    return ((SR * SP + 1) & ~1) + BR + NC + (keep ? NC * LDL : 0);
+    return ((SR * SP + 1) & ~1) + BR + XS + (keep ? NC * LDL + NC / 2 : 0);
   }
 };
+
+// Rows r0 .. r0+nr-1 of the row-major n x n matrix A into the staging tile (pitch SP): the rows
+// are contiguous in A, so lane l loads elements l, l+64, ... of that run (coalesced, all loads in
+// flight together) and scatters them to (row, column) in LDS.  Columns n .. (n+7)&~7 of each
+// staged row are set to +0.
+template <int SR, int SP, int NC>
+__device__ __forceinline__ void wave_stage_rows (const double *A, int n, int r0, int nr, int lane, double *stg)
+{
+  constexpr int TMAX = (SR * NC + 63) / 64;
+  const int cnt = nr * n;
+  const double *src = A + (size_t) r0 * n;
+  const float rn = 1.0f / (float) n;
+  double v[TMAX];
+#pragma unroll
+  for (int t = 0; t < TMAX; t++)
+  {
+    const int f = lane + 64 * t;
+    v[t] = f < cnt ? src[f] : 0.0;
+  }
+#pragma unroll
+  for (int t = 0; t < TMAX; t++)
+  {
+    int f = lane + 64 * t;
+    // recomputed in every pass (the same in each): kept live across passes, the scatter
+    // addresses would be spilled to scratch memory
+    __asm__ volatile ("" : "+v"(f));
+    if (f < cnt)
+    {
+      // f / n: (f + 0.5) / n is at least 0.5 / n from an integer, far above the float error
+      const int rr = (int) (((float) f + 0.5f) * rn);
+      stg[rr * SP + (f - rr * n)] = v[t];
+    }
+  }
+  const int n8 = min ((n + 7) & ~7, SP);
+  if (lane < nr)
+    for (int c = n; c < n8; c++)
+      stg[lane * SP + c] = 0.0;
+}
 
 template <int NC, int WPB, bool KEEP>
 __global__ __launch_bounds__ (64 * WPB, NC == 64 ? 3 : 4) void k_wave_f64 (SolveArgs s)
@@ -208,12 +253,13 @@ __global__ __launch_bounds__ (64 * WPB, NC == 64 ? 3 : 4) void k_wave_f64 (Solve
   using Lay = WaveF64Layout<NC>;
   constexpr int SR = Lay::SR, SP = Lay::SP, LDL = Lay::LDL;
   constexpr int PER_WAVE = (Lay::base (KEEP) + 1) & ~1;
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wv = uni (threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int neff = list_len (s);
   double *stg = smem + wv * PER_WAVE;
   double *brow = stg + ((SR * SP + 1) & ~1);
   double *xs = brow + Lay::BR;
-  double *Ls = xs + NC;                               // KEEP: Ls[row * LDL + k] = multiplier
+  double *Ls = xs + Lay::XS;                          // KEEP: Ls[row * LDL + k] = multiplier
+  int *pvs = reinterpret_cast<int *> (Ls + NC * LDL); // KEEP: GSL's ipiv
   for (int item = blockIdx.x * WPB + wv; item < neff; item += gridDim.x * WPB)
   {
     wave_lds_fence ();
@@ -228,7 +274,7 @@ __global__ __launch_bounds__ (64 * WPB, NC == 64 ? 3 : 4) void k_wave_f64 (Solve
       continue;
     }
 
-    // coalesced load of A, SR rows at a time, through the staging tile
+    // ---- row `lane` of A into registers (+0 past column n) ----
     const bool in = lane < n;
     double a[NC];
 #pragma unroll
@@ -237,38 +283,43 @@ __global__ __launch_bounds__ (64 * WPB, NC == 64 ? 3 : 4) void k_wave_f64 (Solve
     for (int r0 = 0; r0 < n; r0 += SR)
     {
       const int nr = min (SR, n - r0);
-      for (int r = 0; r < nr; r++)
-        for (int c = lane; c < n; c += 64)
-          stg[r * SP + c] = A[(size_t) (r0 + r) * n + c];
+      wave_stage_rows<SR, SP, NC> (A, n, r0, nr, lane, stg);
       wave_lds_fence ();
       if (lane >= r0 && lane < r0 + nr)
       {
 #pragma unroll
         for (int j = 0; j < NC; j++)
-          if (j < n)
-            a[j] = stg[(lane - r0) * SP + j];
+          a[j] = j < n ? stg[(lane - r0) * SP + j] : 0.0;
       }
       wave_lds_fence ();
     }
-    double bb = in ? b[lane] : 0.0;
+    const double b0 = in ? b[lane] : 0.0;
+    double bb = b0;
 
+    // ---- elimination, forward solve folded in ----
     bool alive = in;
     int pos = lane;
     int singular = 0;
     for (int k = 0; k < n; k++)
     {
-      const int q = wave_pivot_lane (alive, pivot_key (a[0]), pos);
+      const int q = max (wave_pivot_lane (alive, pivot_key (a[0]), pos), 0);
       const int ppos = __builtin_amdgcn_readlane (pos, q);
+      const double piv = rl_f64 (a[0], q);
+      const int live = n - k;                          // live columns a[0 .. live-1]
       if (lane == q)
       {
 #pragma unroll
-        for (int j = 0; j < NC; j += 2)
-          if (j < n - k)
-            *reinterpret_cast<double2 *> (brow + j) = make_double2 (a[j], a[j + 1]);
+        for (int g = 0; g < NC; g += 8)
+          if (g < live)
+          {
+#pragma unroll
+            for (int e = 0; e < 8; e += 2)
+              *reinterpret_cast<double2 *> (brow + g + e) = make_double2 (a[g + e], a[g + e + 1]);
+          }
         brow[NC] = bb;
       }
-      if (KEEP && lane == 0 && s.piv)
-        s.piv[s.offV[cell] + k] = ppos;
+      if (KEEP && lane == 0)
+        pvs[k] = ppos;
       if (alive && pos == k)
         pos = ppos;
       if (lane == q)
@@ -276,9 +327,8 @@ __global__ __launch_bounds__ (64 * WPB, NC == 64 ? 3 : 4) void k_wave_f64 (Solve
         alive = false;        // from here on this lane holds U(k, k..) and y_k
         pos = k;
       }
-      wave_lds_fence ();
-      const double piv = brow[0];
       singular |= (piv == 0.0);
+      __asm__ volatile ("" ::: "memory");    // broadcast row written before it is read (LDS order)
       if (alive)
       {
         const double l = gsl_multiplier (a[0], piv);
@@ -286,10 +336,25 @@ __global__ __launch_bounds__ (64 * WPB, NC == 64 ? 3 : 4) void k_wave_f64 (Solve
           Ls[lane * LDL + k] = l;
         bb = bb - l * brow[NC];
 #pragma unroll
-        for (int j = 1; j < NC; j++)
-          if (j < n - k)
-            a[j - 1] = a[j] - l * brow[j];
+        for (int g = 0; g < NC; g += 8)
+          if (g < live)
+          {
+            double u8[8];
+#pragma unroll
+            for (int e = 0; e < 8; e += 2)
+            {
+              const double2 t = *reinterpret_cast<const double2 *> (brow + g + e);
+              u8[e] = t.x;
+              u8[e + 1] = t.y;
+            }
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+              if (g + e >= 1)
+                a[g + e - 1] = a[g + e] - l * u8[e];
+          }
+        a[NC - 1] = 0.0;
       }
+      __asm__ volatile ("" ::: "memory");    // reads of this step before the next step's writes
     }
     wave_lds_fence ();
 
@@ -304,6 +369,9 @@ __global__ __launch_bounds__ (64 * WPB, NC == 64 ? 3 : 4) void k_wave_f64 (Solve
         if (t < n - pos)
           out[pos + t] = a[t];
     }
+    if (KEEP && s.piv)
+      for (int t = lane; t < n; t += 64)
+        s.piv[s.offV[cell] + t] = pvs[t];
     if (singular)
     {
       if (lane == 0)
@@ -311,40 +379,63 @@ __global__ __launch_bounds__ (64 * WPB, NC == 64 ? 3 : 4) void k_wave_f64 (Solve
       continue;
     }
 
-    // back substitution in dtrsv order: the lane holding U row i forms its chain from x in LDS
+    // ---- back substitution in dtrsv order: the lane holding U row i subtracts its products in
+    //      ascending column order from x in LDS (+0 from column n on), then divides ----
+    if (lane < 8)
+      xs[n + lane] = 0.0;
     for (int i = n - 1; i >= 0; i--)
     {
+      __asm__ volatile ("" ::: "memory");    // x_{i+1} (another lane) written before it is read
       if (in && pos == i)
       {
         double tmp = bb;
 #pragma unroll
-        for (int t = 1; t < NC; t++)
-        {
-          if (t < n - i)
+        for (int g = 0; g < NC; g += 8)
+          if (g < n - i)
           {
-            const double pr = a[t] * xs[i + t];
-            tmp = tmp - pr;
+            double x8[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+              x8[e] = xs[i + g + e];
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+              if (g + e >= 1)
+              {
+                const double pr = a[g + e] * x8[e];
+                tmp = tmp - pr;
+              }
           }
-        }
         xs[i] = tmp / a[0];
       }
     }
     wave_lds_fence ();
 
-    // residual check t = A x (gsl_blas_dgemv order) and codes 2 / 3 of the first failing row
+    // ---- residual check t = A x (gsl_blas_dgemv order: 0 + sum in ascending j) and codes 2 / 3
+    //      of the first failing row; A is re-read through the staging tile ----
     int fail = 0;
-    if (in)
+    for (int r0 = 0; r0 < n; r0 += SR)
     {
-      const double *Ai = A + (size_t) lane * n;
-      double t = 0.0;
-#pragma unroll 8
-      for (int j = 0; j < n; j++)
+      const int nr = min (SR, n - r0);
+      wave_stage_rows<SR, SP, NC> (A, n, r0, nr, lane, stg);
+      wave_lds_fence ();
+      if (lane >= r0 && lane < r0 + nr)
       {
-        const double pr = Ai[j] * xs[j];
-        t = t + pr;
+        const double *Ai = stg + (lane - r0) * SP;
+        double t = 0.0;
+#pragma unroll
+        for (int g = 0; g < NC; g += 8)
+          if (g < n)
+          {
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+            {
+              const double pr = Ai[g + e] * xs[g + e];   // +0 * +0 past column n
+              t = t + pr;
+            }
+          }
+        fail = residual_row_code (0.0 + t, b0);
       }
-      const double tv = 0.0 + t;
-      fail = residual_row_code (tv, b[lane]);
+      wave_lds_fence ();
     }
     const unsigned long long fb = __ballot (fail != 0);
     int code = 0;
