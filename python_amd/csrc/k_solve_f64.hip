@@ -86,22 +86,33 @@ __device__ __forceinline__ double sub_oct (double t, const Oct &o)
 // in LDS, 16-byte aligned, and holds +0.0 at every index below j (down to j rounded to even) and
 // from n up to n rounded up to a multiple of eight: the chain runs over whole groups of eight,
 // and subtracting +0.0 leaves every value unchanged (IEEE: t - (+0) == t, -0 and NaN included),
-// so the padding changes nothing.  The next group's reads are in flight while the current group
-// is subtracted.
+// so the padding changes nothing.  Three groups are in flight in three register sets that are
+// each reloaded right after their subtractions (no register copies, so no wait on a just-issued
+// read); the reads run up to 40 doubles past the last group (p must be readable up to n + 48).
 __device__ __forceinline__ double chain_sub (double tmp, const double *p, int j, int n)
 {
   int g = j & ~1;
   if (g >= n)
     return tmp;
+  const int ng = (n - g + 7) >> 3;   // groups of eight to subtract
   Oct c = load_oct (p + g);
-  for (; g + 8 < n; g += 8)
+  Oct d = load_oct (p + g + 8);
+  Oct e = load_oct (p + g + 16);
+  int t = 0;
+  for (; t + 3 <= ng; t += 3, g += 24)
   {
-    const Oct d = load_oct (p + g + 8);
-    __builtin_amdgcn_sched_barrier (0);
     tmp = sub_oct (tmp, c);
-    c = d;
+    c = load_oct (p + g + 24);
+    tmp = sub_oct (tmp, d);
+    d = load_oct (p + g + 32);
+    tmp = sub_oct (tmp, e);
+    e = load_oct (p + g + 40);
   }
-  return sub_oct (tmp, c);
+  if (t < ng)
+    tmp = sub_oct (tmp, c);
+  if (t + 1 < ng)
+    tmp = sub_oct (tmp, d);
+  return tmp;
 }
 
 // Back substitution U x = y in dtrsv order, run by one wave.  Row i of U is at U + i*ldu
@@ -110,7 +121,7 @@ __device__ __forceinline__ double chain_sub (double tmp, const double *p, int j,
 // 16-byte aligned, zeros elsewhere); the wave then subtracts them in ascending j (the
 // chain of dependent FP64 subtractions of GSL's dtrsv, cblas/source_trsv_r.h:35-52) and divides
 // by U_ii.  Four rows are in flight: the row loaded into a slot is used four rows later, so the
-// short chains near the bottom of U do not wait on memory.  xs needs n + 8 doubles.  On return
+// short chains near the bottom of U do not wait on memory.  xs needs n + 48 doubles.  On return
 // xs holds x.
 template <int MAXM>
 __device__ __forceinline__ void wave_backsub (const double *U, size_t ldu, const double *y, size_t ys, int n, int lane,
@@ -157,19 +168,19 @@ __device__ __forceinline__ void wave_backsub (const double *U, size_t ldu, const
         for (int m = 0; m < MAXM; m++)
           if (lane + 64 * m == i)
             xr[m] = xi;
-        const int r = i - D;       // refill this slot with the row it serves next
-        if (r >= 0)
-        {
-#pragma unroll
-          for (int m = 0; m < MAXM; m++)
-          {
-            const int j = lane + 64 * m;
-            cu[sl][m] = j < n ? U[(size_t) r * ldu + j] : 0.0;
-          }
-          cy[sl] = y[(size_t) r * ys];
-          cd[sl] = U[(size_t) r * ldu + r];
-        }
       }
+      // refill this slot with the row it serves next, unconditionally (row 0 again past the
+      // top): a conditional refill would merge old and new values, and the register copies
+      // that merge needs would wait for the loads just issued
+      const int r = max (i - D, 0);
+#pragma unroll
+      for (int m = 0; m < MAXM; m++)
+      {
+        const int j = lane + 64 * m;
+        cu[sl][m] = j < n ? U[(size_t) r * ldu + j] : 0.0;
+      }
+      cy[sl] = y[(size_t) r * ys];
+      cd[sl] = U[(size_t) r * ldu + r];
     }
   }
 #pragma unroll
@@ -785,7 +796,7 @@ __global__ __launch_bounds__ (256, MAXM <= 4 ? 4 : 1) void k_bsub_f64 (SolveArgs
 {
   extern __shared__ double smem[];
   constexpr int WPB = 4;
-  constexpr int XP = 64 * MAXM + 16;     // per-wave product / x staging (n + 7 readable)
+  constexpr int XP = 64 * MAXM + 48;     // per-wave product / x staging (n + 47 readable)
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane (threadIdx.x >> 6);
   double *xs = smem + wv * XP;
@@ -923,7 +934,7 @@ static void launch_bsub_m (const SolveArgs &a, hipStream_t st)
 {
   const int full = (a.nlist + 3) / 4;
   const int grid = std::max (1, std::min (full, 4 * hipla_device_info ().num_cu));
-  const size_t sm = 4 * (64 * MAXM + 16) * sizeof (double);
+  const size_t sm = 4 * (64 * MAXM + 48) * sizeof (double);
   hipLaunchKernelGGL ((k_bsub_f64<MAXM>), dim3 (grid), dim3 (256), sm, st, a);
 }
 
