@@ -596,9 +596,14 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
           combine_pivot_candidates (c_key[buf], c_pos[buf], c_row[buf], NW, lane, pq, p);
           const double *u = c_val[buf][(p & (BS - 1)) >> 6];
           const double piv = u[kk];
+          // the pivot row's remaining columns, read together (one LDS wait, not one per column);
+          // columns from nbk on are zero padding in every row and are never written back
+          double uu[NB + 1];
+#pragma unroll
+          for (int c = kk + 1; c <= NB; c++)
+            uu[c] = u[c];
           singular |= (piv == 0.0);
           const bool recip = fabs (piv) >= DBL_MIN;
-          const double sc = 1.0 / piv;
 #pragma unroll
           for (int pr = 0; pr < PR; pr++)
           {
@@ -606,17 +611,14 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
             // every row still below the pivot, the row at position kk (moving to pq) included
             if (r != p && mp[pr] >= kk && mp[pr] < rows)
             {
-              double l;
-              if (recip)
-                l = v[pr][kk] * sc;
-              else
-                l = v[pr][kk] / piv;
+              // GSL's multiplier: reciprocal multiply, or a division for a tiny pivot
+              const double t = (recip ? 1.0 : v[pr][kk]) / piv;
+              const double l = recip ? v[pr][kk] * t : t;
               v[pr][kk] = l;
 #pragma unroll
               for (int c = kk + 1; c < NB; c++)
-                if (c < nbk)
-                  v[pr][c] = v[pr][c] - l * u[c];
-              v[pr][NB] = v[pr][NB] - l * u[NB];
+                v[pr][c] = v[pr][c] - l * uu[c];
+              v[pr][NB] = v[pr][NB] - l * uu[NB];
             }
             // GSL swaps the rows at positions kk and pq
             if (r == p)
@@ -737,12 +739,31 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
       __syncthreads ();
 
       // ---- 3. A22 -= L21 U12 (only when the panel is full) ----
+      // Tiles of RB rows x 64 columns, one column per lane, NB updates per element in ascending
+      // kk.  The next tile's A22 values are loaded before the current tile is updated (rows past
+      // n are clamped to row n-1 for the loads and not stored).
       const int nr = n - k1;
       if (nr > 0)
       {
         const int nch = (nr + 63) >> 6;
         const int nit = nch * ((nr + RB - 1) / RB);
-        for (int it = wv; it < nit; it += NW)
+        double an[RB];
+#pragma unroll
+        for (int r = 0; r < RB; r++)
+          an[r] = 0.0;
+        int it = wv;
+        if (it < nit)
+        {
+          const int c = k1 + (it % nch) * 64 + lane;
+          const int i0 = k1 + (it / nch) * RB;
+          if (c < n)
+          {
+#pragma unroll
+            for (int r = 0; r < RB; r++)
+              an[r] = wrow (W, n, min (i0 + r, n - 1))[(unsigned) c];
+          }
+        }
+        for (; it < nit; it += NW)
         {
           const int ch = it % nch;
           const int i0 = k1 + (it / nch) * RB;
@@ -752,7 +773,19 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
           double a[RB];
 #pragma unroll
           for (int r = 0; r < RB; r++)
-            a[r] = (cv && i0 + r < n) ? wrow (W, n, i0 + r)[cu] : 0.0;
+            a[r] = an[r];
+          const int nx = it + NW;
+          if (nx < nit)
+          {
+            const int cn = k1 + (nx % nch) * 64 + lane;
+            const int in0 = k1 + (nx / nch) * RB;
+            if (cn < n)
+            {
+#pragma unroll
+              for (int r = 0; r < RB; r++)
+                an[r] = wrow (W, n, min (in0 + r, n - 1))[(unsigned) cn];
+            }
+          }
           double u[NB];
 #pragma unroll
           for (int kk = 0; kk < NB; kk++)
