@@ -20,6 +20,7 @@ hipError_t launch_wave_f64 (const SolveArgs &a, int nc, bool keep, int grid, hip
 // (k_tile.hip) + back substitution
 hipError_t launch_wgr_f64 (const SolveArgs &a, int nmax, bool keep, int grid, hipStream_t st);
 hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, bool keep, hipStream_t st);
+int blocked_f64_wgs_per_cu (int nmax);   // resident workgroups per CU (persistent grid size)
 hipError_t setup_solve_f64_attrs (size_t max_smem);
 int wgr_f64_limit ();   // largest n of the workgroup-resident FP64 route
 int tile_f64_limit ();

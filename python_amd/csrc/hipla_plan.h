@@ -85,9 +85,8 @@ struct hipla_plan_s
   double *d_F = nullptr;      // factor store of the workgroup kernels (d_LU when factors are kept)
   bool own_F = false;
   double *d_Y = nullptr;      // forward-substituted right-hand sides (B's layout)
-  // fork-join streams / events
+  // the plan's own stream (graph capture, tuning)
   std::vector<hipStream_t> aux;
-  std::vector<hipEvent_t> ev;
   // graph cache
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -98,7 +97,7 @@ struct hipla_plan_s
   int solved = 0;
 };
 
-// launch every group of a solve / invert on `st` (fork-join over aux streams)
+// launch every group of a solve / invert on `st`, one after another
 int plan_enqueue_solve (hipla_plan p, const double *dA, const double *dB, double *dX, int *dcodes,
                         hipla_cell_stats *dstats, hipStream_t st);
 int plan_enqueue_mixed (hipla_plan p, const double *dA, const double *dB, double *dX, int *dcodes,

@@ -84,10 +84,7 @@ static int free_plan_device (hipla_plan p)
       (void) hipFree (b);
   for (auto s : p->aux)
     (void) hipStreamDestroy (s);
-  for (auto e : p->ev)
-    (void) hipEventDestroy (e);
   p->aux.clear ();
-  p->ev.clear ();
   return 0;
 }
 
@@ -173,7 +170,7 @@ extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, con
     // FP64 kernel; it factors each cell in place in the plan's factor store
     if (g.kind == KK_BLOCKED_F64 || g.kind == KK_BLOCKED_MIX ||
         (mode == HIPLA_MODE_MIXED && g.nmax > wgr_f64_limit ()))
-      g.grid = std::min (g.count, 2 * dev.num_cu);
+      g.grid = std::min (g.count, blocked_f64_wgs_per_cu (g.nmax) * dev.num_cu);
     p->groups.push_back (g);
     i = j;
   }
@@ -245,14 +242,8 @@ extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, con
       PCHECK (hipMalloc (&p->d_Y, sizeof (double) * p->totV));
     }
   }
-  const int nbranch = (int) p->groups.size ();
-  p->aux.resize (nbranch + 1, nullptr);
-  p->ev.resize (nbranch + 1, nullptr);
-  for (int b = 0; b <= nbranch; b++)
-  {
-    PCHECK (hipStreamCreateWithFlags (&p->aux[b], hipStreamNonBlocking));
-    PCHECK (hipEventCreateWithFlags (&p->ev[b], hipEventDisableTiming));
-  }
+  p->aux.resize (1, nullptr);
+  PCHECK (hipStreamCreateWithFlags (&p->aux[0], hipStreamNonBlocking));
 #undef PCHECK
   *out = p;
   return 0;
@@ -363,19 +354,11 @@ int plan_enqueue_solve (hipla_plan p, const double *dA, const double *dB, double
   if (p->mode == HIPLA_MODE_MIXED)
     return plan_enqueue_mixed (p, dA, dB, dX, dcodes, dstats, st);
   const bool per_cell = (p->flags & HIPLA_FLAG_PER_CELL) != 0;
-  const int nb = (int) p->groups.size ();
-  // fork
-  if (nb > 1)
-    HIPLA_CHECK (hipEventRecord (p->ev[nb], st));
-  for (int gi = 0; gi < nb; gi++)
+  // the size classes one after another on the caller's stream: each class's launch already
+  // spreads over every CU, and classes run side by side only contend for LDS and registers
+  // (measured slower than back to back)
+  for (const Group &g : p->groups)
   {
-    const Group &g = p->groups[gi];
-    hipStream_t s = st;
-    if (nb > 1 && gi > 0)
-    {
-      s = p->aux[gi];
-      HIPLA_CHECK (hipStreamWaitEvent (s, p->ev[nb], 0));
-    }
     SolveArgs a = make_args (p, dA, dB, dX, dcodes, g);
     if (per_cell)
     {
@@ -385,17 +368,12 @@ int plan_enqueue_solve (hipla_plan p, const double *dA, const double *dB, double
         SolveArgs a1 = a;
         a1.list = a.list + c;
         a1.nlist = 1;
-        HIPLA_CHECK (launch_group_f64 (p, a1, g, s));
+        HIPLA_CHECK (launch_group_f64 (p, a1, g, st));
       }
     }
     else
-      HIPLA_CHECK (launch_group_f64 (p, a, g, s));
-    if (nb > 1 && gi > 0)
-      HIPLA_CHECK (hipEventRecord (p->ev[gi], s));
+      HIPLA_CHECK (launch_group_f64 (p, a, g, st));
   }
-  // join
-  for (int gi = 1; gi < nb; gi++)
-    HIPLA_CHECK (hipStreamWaitEvent (st, p->ev[gi], 0));
   return 0;
 }
 

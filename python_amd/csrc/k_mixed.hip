@@ -1005,36 +1005,60 @@ int enqueue_mixed_fallback (hipla_plan p, const double *dA, const double *dB, do
     else if (r == 3)
       HIPLA_CHECK (launch_wgr_f64 (a, std::min (nmax_all, wgr_f64_limit ()), false, dev.num_cu, st));
     else
-      HIPLA_CHECK (launch_blocked_f64 (a, nmax_all, dev.num_cu, false, st));
+      HIPLA_CHECK (launch_blocked_f64 (a, nmax_all, blocked_f64_wgs_per_cu (nmax_all) * dev.num_cu, false, st));
   }
   return 0;
+}
+
+// FP64 kernel family of a group (wave 16 / 32 / 64, workgroup tile, blocked)
+static int fp64_family (int nmax)
+{
+  return nmax <= 16 ? 0 : nmax <= 32 ? 1 : nmax <= 64 ? 2 : nmax <= wgr_f64_limit () ? 3 : 4;
 }
 
 int plan_enqueue_mixed (hipla_plan p, const double *dA, const double *dB, double *dX, int *dcodes,
                         hipla_cell_stats *dstats, hipStream_t st)
 {
   const int nb = (int) p->groups.size ();
-  int rc = enqueue_mixed_reset (p, st);
+  // no group on the FP32 path: no cell can be handed back, so no fallback list to clear or solve
+  bool any_mixed = false;
+  for (const Group &g : p->groups)
+    any_mixed |= !(g.kind == KK_BLOCKED_MIX || g.direct);
+  int rc = any_mixed ? enqueue_mixed_reset (p, st) : 0;
   if (rc)
     return rc;
-  if (nb > 1)
-    HIPLA_CHECK (hipEventRecord (p->ev[nb], st));
-  for (int gi = 0; gi < nb; gi++)
+  // Groups one after another on the caller's stream.  Adjacent groups that go straight to the
+  // same FP64 kernel (their cells are contiguous in the plan's list) are solved by one launch, so
+  // a persistent grid covers all of them at once.
+  const DeviceInfo &dev = hipla_device_info ();
+  for (int gi = 0; gi < nb;)
   {
-    hipStream_t s = st;
-    if (nb > 1 && gi > 0)
+    const Group &g = p->groups[gi];
+    if (!(g.kind == KK_BLOCKED_MIX || g.direct))
     {
-      s = p->aux[gi];
-      HIPLA_CHECK (hipStreamWaitEvent (s, p->ev[nb], 0));
+      if ((rc = enqueue_mixed_group (p, gi, dA, dB, dX, dcodes, dstats, st)))
+        return rc;
+      gi++;
+      continue;
     }
-    if ((rc = enqueue_mixed_group (p, gi, dA, dB, dX, dcodes, dstats, s)))
+    Group u = g;
+    int gj = gi + 1;
+    for (; gj < nb; gj++)
+    {
+      const Group &h = p->groups[gj];
+      if (!(h.kind == KK_BLOCKED_MIX || h.direct) || fp64_family (h.nmax) != fp64_family (u.nmax) ||
+          h.first != u.first + u.count)
+        break;
+      u.count += h.count;
+      u.nmax = std::max (u.nmax, h.nmax);
+    }
+    u.grid = std::min (u.count, blocked_f64_wgs_per_cu (u.nmax) * dev.num_cu);
+    SolveArgs a = group_args (p, dA, dB, dX, dcodes, u);
+    if ((rc = launch_group_direct_fp64 (p, a, u, dstats, st)))
       return rc;
-    if (nb > 1 && gi > 0)
-      HIPLA_CHECK (hipEventRecord (p->ev[gi], s));
+    gi = gj;
   }
-  for (int gi = 1; gi < nb; gi++)
-    HIPLA_CHECK (hipStreamWaitEvent (st, p->ev[gi], 0));
-  return enqueue_mixed_fallback (p, dA, dB, dX, dcodes, st);
+  return any_mixed ? enqueue_mixed_fallback (p, dA, dB, dX, dcodes, st) : 0;
 }
 
 hipError_t setup_mixed_attrs (size_t)

@@ -504,8 +504,7 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
     double *wb = s.Y + s.offV[cell];
     double *P = smem;
     double *Us = P + (size_t) n * PP;
-    double *xs = Us + (US ? (size_t) NB * n : 0);
-    int *pos = (int *) (xs + n);       // position of panel row r (both relative to k0)
+    int *pos = (int *) (Us + (US ? (size_t) NB * n : 0));   // position of panel row r (relative to k0)
     int *phys = pos + n;               // panel row at position q
 
     {
@@ -906,7 +905,8 @@ int wgr_f64_limit ()
 }
 
 // Blocked kernel variants: NB = 16 with U12 staged in LDS up to n = 512, NB = 8 reading U12 from
-// the workspace up to n = 1024 (LDS: panel n x (NB+2), U12 NB x n, x, two int maps of n).
+// the factor store up to n = 1024 (LDS: panel n x (NB+2), U12 NB x n when staged, two int maps
+// of n).
 static void blocked_variant (int nmax, int &nb, bool &us)
 {
   nb = nmax <= 512 ? 16 : 8;
@@ -919,7 +919,14 @@ size_t blocked_f64_smem (int nmax)
   bool us;
   blocked_variant (nmax, nb, us);
   return (size_t) nmax * (nb + 2) * sizeof (double) + (us ? (size_t) nb * nmax * sizeof (double) : 0) +
-         (size_t) nmax * sizeof (double) + 2 * (size_t) nmax * sizeof (int);
+         2 * (size_t) nmax * sizeof (int);
+}
+
+// Workgroups of the blocked kernel one CU holds at once (LDS and registers), for the persistent
+// grid: two (about 74 KB of LDS and 4 x 128 registers per SIMD each at n = 256).
+int blocked_f64_wgs_per_cu (int)
+{
+  return 2;
 }
 
 template <int NC, int WPB>
@@ -987,13 +994,13 @@ static hipError_t launch_bsub_f64 (const SolveArgs &a, int nmax, hipStream_t st)
   return hipGetLastError ();
 }
 
-template <int NB, bool US>
+template <int BS, int NB, bool US>
 static void launch_blocked_m (const SolveArgs &a, int grid, size_t sm, bool keep, hipStream_t st)
 {
   if (keep)
-    hipLaunchKernelGGL ((k_blocked_f64<BLK_BS, NB, US, true>), dim3 (grid), dim3 (BLK_BS), sm, st, a);
+    hipLaunchKernelGGL ((k_blocked_f64<BS, NB, US, true>), dim3 (grid), dim3 (BS), sm, st, a);
   else
-    hipLaunchKernelGGL ((k_blocked_f64<BLK_BS, NB, US, false>), dim3 (grid), dim3 (BLK_BS), sm, st, a);
+    hipLaunchKernelGGL ((k_blocked_f64<BS, NB, US, false>), dim3 (grid), dim3 (BS), sm, st, a);
 }
 
 hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, bool keep, hipStream_t st)
@@ -1004,9 +1011,9 @@ hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, bool keep
     return hipErrorInvalidValue;
   const size_t sm = blocked_f64_smem (nmax);
   if (nmax <= 512)
-    launch_blocked_m<16, true> (a, grid, sm, keep, st);
+    launch_blocked_m<BLK_BS, 16, true> (a, grid, sm, keep, st);
   else
-    launch_blocked_m<8, false> (a, grid, sm, keep, st);
+    launch_blocked_m<BLK_BS, 8, false> (a, grid, sm, keep, st);
   hipError_t e = hipGetLastError ();
   if (e != hipSuccess)
     return e;

@@ -94,10 +94,10 @@ def test_mixed_singular_goes_to_fp64(hip, oracle_lib):
     assert np.all(stats["fallback"] == 1)
 
 
-def test_mixed_autotuned_classes_run_concurrently(hip, oracle_lib):
-    """After autotuning, size classes routed straight to the FP64 kernels run concurrently on
-    their own streams (and under graph replay); the blocked ones each use their own workspace
-    slots, so every cell still equals the FP64 oracle and the codes match."""
+def test_mixed_autotuned_fp64_classes_merged(hip, oracle_lib):
+    """After autotuning, adjacent size classes routed straight to the same FP64 kernel are solved
+    by one launch (also under graph replay); every such cell still equals the FP64 oracle and the
+    codes match."""
     import torch
     from python_amd.generators import uneven
     batch = uneven(scale=0.06)        # 600 cells: 30 each of n = 200 and n = 256
