@@ -52,6 +52,10 @@ __device__ __forceinline__ double *wrow (double *W, int n, int row)
 {
   return W + (size_t) (unsigned) row * (unsigned) n;
 }
+__device__ __forceinline__ const double *wrow (const double *W, int n, int row)
+{
+  return W + (size_t) (unsigned) row * (unsigned) n;
+}
 
 // Eight consecutive products p[g .. g+7] (g even: 16-byte aligned reads).
 struct Oct
@@ -507,25 +511,6 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
     int *pos = (int *) (Us + (US ? (size_t) NB * n : 0));   // position of panel row r (relative to k0)
     int *phys = pos + n;               // panel row at position q
 
-    {
-      // eight loads in flight per thread, then their stores
-      const int64_t tot = (int64_t) n * n;
-      int64_t e = tid;
-      for (; e + 7 * BS < tot; e += 8 * BS)
-      {
-        double t[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++)
-          t[u] = A[e + u * BS];
-#pragma unroll
-        for (int u = 0; u < 8; u++)
-          W[e + u * BS] = t[u];
-      }
-      for (; e < tot; e += BS)
-        W[e] = A[e];
-    }
-    for (int i = tid; i < n; i += BS)
-      wb[i] = b[i];
     if (tid == 0)
       s_fail = 0x7fffffff;
     __syncthreads ();
@@ -536,6 +521,10 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
       const int nbk = min (NB, n - k0);
       const int rows = n - k0;
       const int k1 = k0 + nbk;
+      // the first panel reads A and b themselves (no copy into the factor store): everything it
+      // does not write lands in the factor store with the first trailing update
+      const double *S = k0 == 0 ? A : W;
+      const double *Sb = k0 == 0 ? b : wb;
       // ---- 1. panel factorization in registers: thread t holds panel rows t + BS pr ----
       // Each step publishes, per wave, the best candidate (GSL key, position) together with
       // that row's values, so after one barrier every thread can eliminate against the winner.
@@ -546,11 +535,11 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
       {
         const int r = tid + BS * pr;
         mp[pr] = r < rows ? r : 0x7fffffff;
-        const double *src = W + (size_t) (k0 + min (r, rows - 1)) * n + k0;
+        const double *src = S + (size_t) (k0 + min (r, rows - 1)) * n + k0;
 #pragma unroll
         for (int c = 0; c < NB; c++)
           v[pr][c] = (r < rows && c < nbk) ? src[c] : 0.0;
-        v[pr][NB] = r < rows ? wb[k0 + r] : 0.0;
+        v[pr][NB] = r < rows ? Sb[k0 + r] : 0.0;
       }
       if (tid == 0)
         s_nmv = 0;
@@ -680,7 +669,7 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
         // all loads first (indices clamped so every one is in bounds, conditions wave-uniform)
 #pragma unroll
         for (int kk = 0; kk < NB; kk++)
-          u[kk] = wrow (W, n, k0 + uni (phys[min (kk, last)]))[cu];
+          u[kk] = wrow (S, n, k0 + uni (phys[min (kk, last)]))[cu];
         if (US)
         {
           // park the permuted rows in this column's U12 slots (free until written below) so the
@@ -695,7 +684,7 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
           double v[NB];
 #pragma unroll
           for (int m = 0; m < NB; m++)
-            v[m] = wrow (W, n, k0 + uni (mv_src[min (m, nmv - 1)]))[cu];
+            v[m] = wrow (S, n, k0 + uni (mv_src[min (m, nmv - 1)]))[cu];
 #pragma unroll
           for (int m = 0; m < NB; m++)
             if (m < nmv)
@@ -746,6 +735,11 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
       {
         const int nch = (nr + 63) >> 6;
         const int nit = nch * ((nr + RB - 1) / RB);
+        // A22 rows: from A in the first panel unless the row permutation moved them (then 2b
+        // wrote them to the factor store), from the factor store afterwards
+        auto a22 = [&] (int q) -> const double * {
+          return (k0 == 0 && uni (phys[q - k0]) == q - k0) ? wrow (A, n, q) : wrow (W, n, q);
+        };
         double an[RB];
 #pragma unroll
         for (int r = 0; r < RB; r++)
@@ -759,7 +753,7 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
           {
 #pragma unroll
             for (int r = 0; r < RB; r++)
-              an[r] = wrow (W, n, min (i0 + r, n - 1))[(unsigned) c];
+              an[r] = a22 (min (i0 + r, n - 1))[(unsigned) c];
           }
         }
         for (; it < nit; it += NW)
@@ -782,19 +776,23 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
             {
 #pragma unroll
               for (int r = 0; r < RB; r++)
-                an[r] = wrow (W, n, min (in0 + r, n - 1))[(unsigned) cn];
+                an[r] = a22 (min (in0 + r, n - 1))[(unsigned) cn];
             }
           }
           double u[NB];
 #pragma unroll
           for (int kk = 0; kk < NB; kk++)
             u[kk] = cv ? (US ? Us[kk * n + c] : wrow (W, n, k0 + kk)[cu]) : 0.0;
+          int prow[RB];                // the tile rows' panel rows, read together
+#pragma unroll
+          for (int r = 0; r < RB; r++)
+            prow[r] = uni (phys[min (i0 + r, n - 1) - k0]);
 #pragma unroll
           for (int r = 0; r < RB; r++)
           {
             if (i0 + r < n)
             {
-              const double *Li = P + (size_t) uni (phys[i0 + r - k0]) * PP;
+              const double *Li = P + (size_t) prow[r] * PP;
 #pragma unroll
               for (int kk = 0; kk < NB; kk++)
                 a[r] = a[r] - Li[kk] * u[kk];
