@@ -166,7 +166,11 @@ __device__ __forceinline__ void wave_backsub (const double *U, size_t ldu, const
           if (j < n)
             xs[j] = j > i ? cu[sl][m] * xr[m] : 0.0;
         }
+        // the chain reads other lanes' products: to the compiler those are other threads'
+        // stores, so keep it from reusing anything it read from xs before
+        __asm__ volatile ("" ::: "memory");
         const double tmp = chain_sub (cy[sl], xs, i + 1, n);
+        __asm__ volatile ("" ::: "memory");
         const double xi = tmp / cd[sl];
 #pragma unroll
         for (int m = 0; m < MAXM; m++)

@@ -164,8 +164,11 @@ __global__ __launch_bounds__ (64 * NW, 4) void k_tile_f64 (SolveArgs s)
                     s_u[wv][CPW - 1] = a[r][CPW - 1];
                 }
             }
-            // the whole row and this lane's multipliers at once (the pivot lane's writes above
-            // precede these reads in LDS order; s_l of a row that is not updated is not used)
+            // The whole row and this lane's multipliers at once.  The pivot lane's writes precede
+            // these reads in LDS order, but to the compiler they are another thread's stores: the
+            // memory clobber keeps it from forwarding this lane's own (stale) view of s_u.  (s_l of
+            // a row that is not updated is not used.)
+            __asm__ volatile ("" ::: "memory");
             double u[CPW], lv[R];
 #pragma unroll
             for (int c = 0; c + 1 < CPW; c += 2)
@@ -213,6 +216,7 @@ __global__ __launch_bounds__ (64 * NW, 4) void k_tile_f64 (SolveArgs s)
                 for (int c = 0; c < CPW; c++)
                   a[r][c] = a[r][c] - lv[r] * u[c];
               }
+            __asm__ volatile ("" ::: "memory");   // this step's reads before the next step's writes
           }
         }
       }
