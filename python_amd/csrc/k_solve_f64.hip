@@ -93,64 +93,101 @@ __device__ __forceinline__ double sub_oct (double t, const Oct &o)
 // so the padding changes nothing.  Three groups are in flight in three register sets that are
 // each reloaded right after their subtractions (no register copies, so no wait on a just-issued
 // read); the reads run up to 40 doubles past the last group (p must be readable up to n + 48).
-__device__ __forceinline__ double chain_sub (double tmp, const double *p, int j, int n)
+// chain_load issues the first three reads, so that a caller can start them early; chain_run
+// subtracts.
+struct Chain
 {
-  int g = j & ~1;
+  Oct c, d, e;
+  int g;
+};
+
+__device__ __forceinline__ Chain chain_load (const double *p, int j)
+{
+  Chain h;
+  h.g = j & ~1;
+  h.c = load_oct (p + h.g);
+  h.d = load_oct (p + h.g + 8);
+  h.e = load_oct (p + h.g + 16);
+  return h;
+}
+
+__device__ __forceinline__ double chain_run (double tmp, const double *p, Chain h, int n)
+{
+  int g = h.g;
   if (g >= n)
     return tmp;
   const int ng = (n - g + 7) >> 3;   // groups of eight to subtract
-  Oct c = load_oct (p + g);
-  Oct d = load_oct (p + g + 8);
-  Oct e = load_oct (p + g + 16);
   int t = 0;
   for (; t + 3 <= ng; t += 3, g += 24)
   {
-    tmp = sub_oct (tmp, c);
-    c = load_oct (p + g + 24);
-    tmp = sub_oct (tmp, d);
-    d = load_oct (p + g + 32);
-    tmp = sub_oct (tmp, e);
-    e = load_oct (p + g + 40);
+    // the scheduling barriers keep each reload right after the subtractions that free its
+    // registers (left alone, the scheduler sinks all twelve reads to the end of the iteration,
+    // and the next iteration starts by waiting a full LDS round trip)
+    tmp = sub_oct (tmp, h.c);
+    __builtin_amdgcn_sched_barrier (0);
+    h.c = load_oct (p + g + 24);
+    __builtin_amdgcn_sched_barrier (0);
+    tmp = sub_oct (tmp, h.d);
+    __builtin_amdgcn_sched_barrier (0);
+    h.d = load_oct (p + g + 32);
+    __builtin_amdgcn_sched_barrier (0);
+    tmp = sub_oct (tmp, h.e);
+    __builtin_amdgcn_sched_barrier (0);
+    h.e = load_oct (p + g + 40);
+    __builtin_amdgcn_sched_barrier (0);
   }
   if (t < ng)
-    tmp = sub_oct (tmp, c);
+    tmp = sub_oct (tmp, h.c);
   if (t + 1 < ng)
-    tmp = sub_oct (tmp, d);
+    tmp = sub_oct (tmp, h.d);
   return tmp;
 }
 
 // Back substitution U x = y in dtrsv order, run by one wave.  Row i of U is at U + i*ldu
 // (columns i..n-1 at their own column index) and y_i at y[i*ys].  Lane l keeps x_{l + 64 m} in
-// xr[m].  For row i every lane forms its products U_ij x_j in parallel and stages them in xs (LDS,
-// 16-byte aligned, zeros elsewhere); the wave then subtracts them in ascending j (the
-// chain of dependent FP64 subtractions of GSL's dtrsv, cblas/source_trsv_r.h:35-52) and divides
-// by U_ii.  Four rows are in flight: the row loaded into a slot is used four rows later, so the
-// short chains near the bottom of U do not wait on memory.  xs needs n + 48 doubles.  On return
+// xr[m].  Row i subtracts U_ij x_j in ascending j (the chain of dependent FP64 subtractions of
+// GSL's dtrsv, cblas/source_trsv_r.h:35-52) and divides by U_ii.  Its first term U_i,i+1 x_{i+1}
+// needs the x just computed, so it is formed in registers; the products of its later terms only
+// need older x, so the lanes form them in parallel one row ahead (while row i+1's chain runs)
+// and stage them in one of two LDS buffers, where the chain reads them 8 at a time.  Four rows
+// of U are in flight: the row loaded into a slot is used four rows later, so the short chains near
+// the bottom of U do not wait on memory.  xs holds two buffers of XB >= n + 48 doubles; on return
 // xs holds x.
 template <int MAXM>
 __device__ __forceinline__ void wave_backsub (const double *U, size_t ldu, const double *y, size_t ys, int n, int lane,
-                                              double (&xr)[MAXM], double *xs)
+                                              double (&xr)[MAXM], double *xs, int XB)
 {
   constexpr int D = 4;
+  double *bufs0 = xs, *bufs1 = xs + XB;
 #pragma unroll
   for (int m = 0; m < MAXM; m++)
     xr[m] = 0.0;
   if (lane < 8)
-    xs[n + lane] = 0.0;          // padding of the last group (see chain_sub)
-  double cu[D][MAXM], cy[D], cd[D];
+  {
+    bufs0[n + lane] = 0.0;       // padding of the last group (see chain_run)
+    bufs1[n + lane] = 0.0;
+  }
+  // slots: row r's columns (lane's share), y_r, U_rr and U_r,r+1.  A lane past column n-1
+  // reads column n-1 instead and never uses it: with no lane-dependent branch around the
+  // reads, every refill issues the same number of reads, and the waits for a slot's values
+  // stop at that slot's reads instead of at the last rows'.
+  double cu[D][MAXM], cy[D], cd[D], cn[D];
 #pragma unroll
   for (int sl = 0; sl < D; sl++)
   {
-    const int r = max (n - 1 - sl, 0);
+    const int r = uni (max (n - 1 - sl, 0));
+    const double *Ur = U + (size_t) r * ldu;
 #pragma unroll
     for (int m = 0; m < MAXM; m++)
-    {
-      const int j = lane + 64 * m;
-      cu[sl][m] = j < n ? U[(size_t) r * ldu + j] : 0.0;
-    }
+      cu[sl][m] = Ur[min (lane + 64 * m, n - 1)];
     cy[sl] = y[(size_t) r * ys];
     cd[sl] = U[(size_t) r * ldu + r];
+    cn[sl] = U[(size_t) r * ldu + min (r + 1, n - 1)];
+    // slot by slot, in the order the loop refills them: the waits in the loop are counted for
+    // this order too (the count has to hold on entry as well as around the loop)
+    __builtin_amdgcn_sched_barrier (0);
   }
+  double xnew = 0.0;             // x_{i+1}, the newest solution value (wave-uniform)
   for (int i0 = n - 1; i0 >= 0; i0 -= D)
   {
 #pragma unroll
@@ -159,36 +196,48 @@ __device__ __forceinline__ void wave_backsub (const double *U, size_t ldu, const
       const int i = i0 - sl;
       if (i >= 0)
       {
-#pragma unroll
-        for (int m = 0; m < MAXM; m++)
-        {
-          const int j = lane + 64 * m;
-          if (j < n)
-            xs[j] = j > i ? cu[sl][m] * xr[m] : 0.0;
-        }
         // the chain reads other lanes' products: to the compiler those are other threads'
-        // stores, so keep it from reusing anything it read from xs before
+        // stores, so keep it from reusing anything it read from the buffers before
         __asm__ volatile ("" ::: "memory");
-        const double tmp = chain_sub (cy[sl], xs, i + 1, n);
+        // row i's first products (written while row i+1 ran) are read before anything waits
+        // on x_{i+1}
+        const double *pb = (i & 1) ? bufs1 : bufs0;
+        const Chain h = chain_load (pb, i + 2);
+        // row i-1's products of columns i+1 .. n-1 (all x known), +0 at and left of column i and
+        // from column n on (the padding), into the buffer row i-1 reads; row i reads the other
+        // one.  Every lane writes, with no branch (row 0 writes a buffer no row reads).
+        {
+          double *bw = ((i - 1) & 1) ? bufs1 : bufs0;
+#pragma unroll
+          for (int m = 0; m < MAXM; m++)
+          {
+            const int j = lane + 64 * m;
+            bw[j] = (j > i && j < n) ? cu[(sl + 1) % D][m] * xr[m] : 0.0;
+          }
+        }
+        // the first term from registers (+0 for the last row: y - (+0) == y)
+        const double pr = (i + 1 < n) ? cn[sl] * xnew : 0.0;
+        double tmp = cy[sl] - pr;
+        tmp = chain_run (tmp, pb, h, n);
         __asm__ volatile ("" ::: "memory");
-        const double xi = tmp / cd[sl];
+        xnew = tmp / cd[sl];
 #pragma unroll
         for (int m = 0; m < MAXM; m++)
           if (lane + 64 * m == i)
-            xr[m] = xi;
+            xr[m] = xnew;
       }
       // refill this slot with the row it serves next, unconditionally (row 0 again past the
       // top): a conditional refill would merge old and new values, and the register copies
       // that merge needs would wait for the loads just issued
-      const int r = max (i - D, 0);
+      const int r = uni (max (i - D, 0));
+      const double *Ur = U + (size_t) r * ldu;
 #pragma unroll
       for (int m = 0; m < MAXM; m++)
-      {
-        const int j = lane + 64 * m;
-        cu[sl][m] = j < n ? U[(size_t) r * ldu + j] : 0.0;
-      }
+        cu[sl][m] = Ur[min (lane + 64 * m, n - 1)];
       cy[sl] = y[(size_t) r * ys];
       cd[sl] = U[(size_t) r * ldu + r];
+      cn[sl] = U[(size_t) r * ldu + min (r + 1, n - 1)];
+      __builtin_amdgcn_sched_barrier (0);
     }
   }
 #pragma unroll
@@ -826,11 +875,12 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
 // chains share each SIMD, so the serial chains of large matrices overlap instead of idling a
 // whole workgroup.
 template <int MAXM>
-__global__ __launch_bounds__ (256, MAXM <= 4 ? 4 : 1) void k_bsub_f64 (SolveArgs s)
+__global__ __launch_bounds__ (64, MAXM <= 4 ? 2 : 1) void k_bsub_f64 (SolveArgs s)
 {
   extern __shared__ double smem[];
-  constexpr int WPB = 4;
-  constexpr int XP = 64 * MAXM + 48;     // per-wave product / x staging (n + 47 readable)
+  constexpr int WPB = 1;                 // one wave per workgroup: the waves spread over every CU
+  constexpr int XB = 64 * MAXM + 48;     // one product buffer (n + 47 readable)
+  constexpr int XP = 2 * XB;             // per wave: two product buffers; x in the first
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane (threadIdx.x >> 6);
   double *xs = smem + wv * XP;
@@ -842,11 +892,59 @@ __global__ __launch_bounds__ (256, MAXM <= 4 ? 4 : 1) void k_bsub_f64 (SolveArgs
       continue;
     const int n = s.n[cell];
     double xr[MAXM];
-    wave_backsub<MAXM> (s.F + s.offA[cell], (size_t) n, s.Y + s.offV[cell], 1, n, lane, xr, xs);
+    wave_backsub<MAXM> (s.F + s.offA[cell], (size_t) n, s.Y + s.offV[cell], 1, n, lane, xr, xs, XB);
 
-    // residual check t = A x (gsl_blas_dgemv order) and codes 2 / 3 of the first failing row
+#pragma unroll
+    for (int m = 0; m < MAXM; m++)
+      if (lane + 64 * m < n)
+        s.X[s.offV[cell] + lane + 64 * m] = xr[m];
+
+    // residual check t = A x (gsl_blas_dgemv order: t_r = sum over ascending j of A_rj x_j) and
+    // codes 2 / 3 of the first failing row.  Lane l runs the chains of rows l + 64 m side by
+    // side, eight columns at a time, with that block of every row read at once: one wave keeps
+    // 16 KB of A in flight (MAXM = 4) instead of 4 KB, and the residual is bound by memory
+    // latency.  Lanes past row n-1 read row n-1 and drop the result.
     const double *A = s.A + s.offA[cell];
     const double *b = s.B + s.offV[cell];
+    const double *Ar[MAXM];
+#pragma unroll
+    for (int m = 0; m < MAXM; m++)
+      Ar[m] = A + (size_t) min (lane + 64 * m, n - 1) * n;
+    double t[MAXM];
+#pragma unroll
+    for (int m = 0; m < MAXM; m++)
+      t[m] = 0.0;
+    int j0 = 0;
+    for (; j0 + 8 <= n; j0 += 8)
+    {
+      double a[MAXM][8];
+#pragma unroll
+      for (int m = 0; m < MAXM; m++)
+#pragma unroll
+        for (int e = 0; e < 8; e++)
+          a[m][e] = Ar[m][j0 + e];
+#pragma unroll
+      for (int e = 0; e < 8; e++)
+      {
+        const double xj = xs[j0 + e];
+#pragma unroll
+        for (int m = 0; m < MAXM; m++)
+        {
+          const double pr = a[m][e] * xj;
+          t[m] = t[m] + pr;
+        }
+      }
+    }
+    for (; j0 < n; j0++)
+    {
+      const double xj = xs[j0];
+#pragma unroll
+      for (int m = 0; m < MAXM; m++)
+      {
+        const double pr = Ar[m][j0] * xj;
+        t[m] = t[m] + pr;
+      }
+    }
     int first = 0x7fffffff;
 #pragma unroll
     for (int m = 0; m < MAXM; m++)
@@ -854,24 +952,12 @@ __global__ __launch_bounds__ (256, MAXM <= 4 ? 4 : 1) void k_bsub_f64 (SolveArgs
       const int r = lane + 64 * m;
       if (r < n)
       {
-        const double *Ar = A + (size_t) r * n;
-        double t = 0.0;
-#pragma unroll 8
-        for (int j = 0; j < n; j++)
-        {
-          const double pr = Ar[j] * xs[j];
-          t = t + pr;
-        }
-        const int f = residual_row_code (0.0 + t, b[r]);
+        const int f = residual_row_code (0.0 + t[m], b[r]);
         if (f)
           first = min (first, (r << 2) | f);
       }
     }
     first = wave_min_i32_dpp (first);
-#pragma unroll
-    for (int m = 0; m < MAXM; m++)
-      if (lane + 64 * m < n)
-        s.X[s.offV[cell] + lane + 64 * m] = xr[m];
     if (lane == 0)
       s.codes[cell] = (first == 0x7fffffff) ? 0 : (first & 3);
   }
@@ -974,10 +1060,9 @@ hipError_t launch_wgr_f64 (const SolveArgs &a, int nmax, bool keep, int grid, hi
 template <int MAXM>
 static void launch_bsub_m (const SolveArgs &a, hipStream_t st)
 {
-  const int full = (a.nlist + 3) / 4;
-  const int grid = std::max (1, std::min (full, 4 * hipla_device_info ().num_cu));
-  const size_t sm = 4 * (64 * MAXM + 48) * sizeof (double);
-  hipLaunchKernelGGL ((k_bsub_f64<MAXM>), dim3 (grid), dim3 (256), sm, st, a);
+  const int grid = std::max (1, std::min (a.nlist, 8 * hipla_device_info ().num_cu));
+  const size_t sm = 2 * (64 * MAXM + 48) * sizeof (double);
+  hipLaunchKernelGGL ((k_bsub_f64<MAXM>), dim3 (grid), dim3 (64), sm, st, a);
 }
 
 // back substitution + residual check of the cells a workgroup factor kernel just factored
