@@ -529,17 +529,36 @@ __global__ __launch_bounds__ (64 * WPB, NC == 64 ? 3 : 4) void k_wave_f64 (Solve
 //   2. the panel is written back in position order; the net row permutation of the panel (at
 //      most 2 NB moved rows) is applied to every other column as one gather / scatter per column,
 //      and U12 = L11^-1 A12 is formed per column (ascending kk, dtrsm's order) into W and LDS.
-//   3. A22 -= L21 U12 in tiles of RB rows x 64 columns per wave; each element takes its NB
-//      updates in ascending kk with the product rounded first, the order of GSL's recursion.
+//   3. A22 -= L21 U12 in 32 x 32 tiles per wave, 4 x 4 elements per lane; each element takes
+//      its NB updates in ascending kk with the product rounded first, the order of GSL's
+//      recursion.
 // Then the back substitution runs from LDS-staged chunks of U and the workgroup forms the
 // residual check.
+// A22 values of a blocked trailing-update tile: rows i0 + 8 r and columns c0 + 8 c (clamped to
+// n-1).  In the first panel a row the panel's permutation left in place is still only in A;
+// every other row is in the factor store W.
+template <int T>
+__device__ __forceinline__ void blk_load_a22 (double (&d)[T][T], const double *A, const double *W, const int *phys,
+                                              int k0, int n, int i0, int c0)
+{
+#pragma unroll
+  for (int r = 0; r < T; r++)
+  {
+    const int q = min (i0 + 8 * r, n - 1);
+    const bool fromA = k0 == 0 && phys[q - k0] == q - k0;
+    const double *row = (fromA ? A : W) + (size_t) q * n;
+#pragma unroll
+    for (int c = 0; c < T; c++)
+      d[r][c] = row[(unsigned) min (c0 + 8 * c, n - 1)];
+  }
+}
+
 template <int BS, int NB, bool US, bool KEEP>
 __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
 {
   extern __shared__ double smem[];
   constexpr int PP = NB + 2;           // panel row pitch (even: 16-byte aligned rows); column NB = b
   constexpr int NW = BS / 64;
-  constexpr int RB = 8;                // rows per trailing-update tile
   constexpr int PR = US ? 1 : 2;       // panel rows per thread (n <= 512: one, else two)
   __shared__ long long c_key[2][NW];   // per-wave pivot candidates, by step parity
   __shared__ int c_pos[2][NW], c_row[2][NW];
@@ -780,81 +799,85 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
       __syncthreads ();
 
       // ---- 3. A22 -= L21 U12 (only when the panel is full) ----
-      // Tiles of RB rows x 64 columns, one column per lane, NB updates per element in ascending
-      // kk.  The next tile's A22 values are loaded before the current tile is updated (rows past
-      // n are clamped to row n-1 for the loads and not stored).
+      // Wave tiles of 32 x 32 elements: lane (rg, cg) = (lane >> 3, lane & 7) updates the 4 x 4
+      // elements at rows i0 + 8 r and columns c0 + 8 c (i0 = tile row + rg, c0 = tile column +
+      // cg, r, c < 4).  Per update step a lane reads 4 multipliers and 4 U12 values for 16
+      // updates: a quarter of the LDS bytes per update of a tile with one column per lane,
+      // whose multipliers are whole-wave broadcasts (the LDS, not the FP64 pipes, bounded that
+      // one).  Each element takes its NB updates in ascending kk with the product rounded
+      // first.  The next tile's A22 values are read before the current tile is updated; rows
+      // and columns past n are clamped to n-1 for the reads and not stored.
       const int nr = n - k1;
       if (nr > 0)
       {
-        const int nch = (nr + 63) >> 6;
-        const int nit = nch * ((nr + RB - 1) / RB);
-        // A22 rows: from A in the first panel unless the row permutation moved them (then 2b
-        // wrote them to the factor store), from the factor store afterwards
-        auto a22 = [&] (int q) -> const double * {
-          return (k0 == 0 && uni (phys[q - k0]) == q - k0) ? wrow (A, n, q) : wrow (W, n, q);
-        };
-        double an[RB];
-#pragma unroll
-        for (int r = 0; r < RB; r++)
-          an[r] = 0.0;
+        constexpr int T = 4;
+        const int nt = (nr + 31) >> 5;
+        const int nit = nt * nt;
+        const int rg = lane >> 3, cg = lane & 7;
+        double an[T][T];
         int it = wv;
         if (it < nit)
-        {
-          const int c = k1 + (it % nch) * 64 + lane;
-          const int i0 = k1 + (it / nch) * RB;
-          if (c < n)
-          {
-#pragma unroll
-            for (int r = 0; r < RB; r++)
-              an[r] = a22 (min (i0 + r, n - 1))[(unsigned) c];
-          }
-        }
+          blk_load_a22<T> (an, A, W, phys, k0, n, k1 + (it / nt) * 32 + rg, k1 + (it % nt) * 32 + cg);
         for (; it < nit; it += NW)
         {
-          const int ch = it % nch;
-          const int i0 = k1 + (it / nch) * RB;
-          const int c = k1 + ch * 64 + lane;
-          const unsigned cu = (unsigned) c;
-          const bool cv = c < n;
-          double a[RB];
+          const int i0 = k1 + (it / nt) * 32 + rg;
+          const int c0 = k1 + (it % nt) * 32 + cg;
+          double a[T][T];
 #pragma unroll
-          for (int r = 0; r < RB; r++)
-            a[r] = an[r];
+          for (int r = 0; r < T; r++)
+#pragma unroll
+            for (int c = 0; c < T; c++)
+              a[r][c] = an[r][c];
           const int nx = it + NW;
           if (nx < nit)
-          {
-            const int cn = k1 + (nx % nch) * 64 + lane;
-            const int in0 = k1 + (nx / nch) * RB;
-            if (cn < n)
-            {
+            blk_load_a22<T> (an, A, W, phys, k0, n, k1 + (nx / nt) * 32 + rg, k1 + (nx % nt) * 32 + cg);
+          int pl[T];                   // LDS offsets of the tile rows' panel rows
+          unsigned cc[T];              // the lane's columns (clamped)
 #pragma unroll
-              for (int r = 0; r < RB; r++)
-                an[r] = a22 (min (in0 + r, n - 1))[(unsigned) cn];
-            }
-          }
-          double u[NB];
+          for (int r = 0; r < T; r++)
+            pl[r] = phys[min (i0 + 8 * r, n - 1) - k0] * PP;
+#pragma unroll
+          for (int c = 0; c < T; c++)
+            cc[c] = (unsigned) min (c0 + 8 * c, n - 1);
 #pragma unroll
           for (int kk = 0; kk < NB; kk++)
-            u[kk] = cv ? (US ? Us[kk * n + c] : wrow (W, n, k0 + kk)[cu]) : 0.0;
-          int prow[RB];                // the tile rows' panel rows, read together
-#pragma unroll
-          for (int r = 0; r < RB; r++)
-            prow[r] = uni (phys[min (i0 + r, n - 1) - k0]);
-#pragma unroll
-          for (int r = 0; r < RB; r++)
           {
-            if (i0 + r < n)
-            {
-              const double *Li = P + (size_t) prow[r] * PP;
+            double l[T], u[T];
 #pragma unroll
-              for (int kk = 0; kk < NB; kk++)
-                a[r] = a[r] - Li[kk] * u[kk];
+            for (int r = 0; r < T; r++)
+              l[r] = P[pl[r] + kk];
+#pragma unroll
+            for (int c = 0; c < T; c++)
+              u[c] = US ? Us[kk * n + cc[c]] : wrow (W, n, k0 + kk)[cc[c]];
+#pragma unroll
+            for (int r = 0; r < T; r++)
+#pragma unroll
+              for (int c = 0; c < T; c++)
+                a[r][c] = a[r][c] - l[r] * u[c];
+            // one step's operands live at a time (hoisting every step's LDS reads to the top
+            // of the unrolled loop would need NB times the registers)
+            __builtin_amdgcn_sched_barrier (0);
+          }
+          // the updates are done here, for every element: left alone, the compiler sinks each
+          // element's update chain into its conditional store and reads every step's operands
+          // first, far more than the registers hold
+#pragma unroll
+          for (int r = 0; r < T; r++)
+#pragma unroll
+            for (int c = 0; c < T; c++)
+              __asm__ volatile ("" : "+v"(a[r][c]));
+#pragma unroll
+          for (int r = 0; r < T; r++)
+          {
+            const int i = i0 + 8 * r;
+            if (i < n)
+            {
+#pragma unroll
+              for (int c = 0; c < T; c++)
+                if (c0 + 8 * c < n)
+                  W[(size_t) i * n + (unsigned) (c0 + 8 * c)] = a[r][c];
             }
           }
-#pragma unroll
-          for (int r = 0; r < RB; r++)
-            if (cv && i0 + r < n)
-              wrow (W, n, i0 + r)[cu] = a[r];
         }
       }
       __syncthreads ();
