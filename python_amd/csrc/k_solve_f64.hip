@@ -529,37 +529,37 @@ __global__ __launch_bounds__ (64 * WPB, NC == 64 ? 3 : 4) void k_wave_f64 (Solve
 //   2. the panel is written back in position order; the net row permutation of the panel (at
 //      most 2 NB moved rows) is applied to every other column as one gather / scatter per column,
 //      and U12 = L11^-1 A12 is formed per column (ascending kk, dtrsm's order) into W and LDS.
-//   3. A22 -= L21 U12 in 32 x 32 tiles per wave, 4 x 4 elements per lane; each element takes
-//      its NB updates in ascending kk with the product rounded first, the order of GSL's
+//   3. A22 -= L21 U12 in 8 TR x 32 tiles per wave, TR x 4 elements per lane; each element
+//      takes its NB updates in ascending kk with the product rounded first, the order of GSL's
 //      recursion.
 // Then the back substitution runs from LDS-staged chunks of U and the workgroup forms the
 // residual check.
 // A22 values of a blocked trailing-update tile: rows i0 + 8 r and columns c0 + 8 c (clamped to
-// n-1).  In the first panel a row the panel's permutation left in place is still only in A;
+// n-1; r < TR, c < TC).  In the first panel a row the panel's permutation left in place is still only in A;
 // every other row is in the factor store W.
-template <int T>
-__device__ __forceinline__ void blk_load_a22 (double (&d)[T][T], const double *A, const double *W, const int *phys,
+template <int TR, int TC>
+__device__ __forceinline__ void blk_load_a22 (double (&d)[TR][TC], const double *A, const double *W, const int *phys,
                                               int k0, int n, int i0, int c0)
 {
 #pragma unroll
-  for (int r = 0; r < T; r++)
+  for (int r = 0; r < TR; r++)
   {
     const int q = min (i0 + 8 * r, n - 1);
     const bool fromA = k0 == 0 && phys[q - k0] == q - k0;
     const double *row = (fromA ? A : W) + (size_t) q * n;
 #pragma unroll
-    for (int c = 0; c < T; c++)
+    for (int c = 0; c < TC; c++)
       d[r][c] = row[(unsigned) min (c0 + 8 * c, n - 1)];
   }
 }
 
 template <int BS, int NB, bool US, bool KEEP>
-__global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
+__global__ __launch_bounds__ (BS, BS == 256 ? 2 : 4) void k_blocked_f64 (SolveArgs s)
 {
   extern __shared__ double smem[];
   constexpr int PP = NB + 2;           // panel row pitch (even: 16-byte aligned rows); column NB = b
   constexpr int NW = BS / 64;
-  constexpr int PR = US ? 1 : 2;       // panel rows per thread (n <= 512: one, else two)
+  constexpr int PR = US ? 1 : 2;       // panel rows per thread (US: n <= BS, one; else two)
   __shared__ long long c_key[2][NW];   // per-wave pivot candidates, by step parity
   __shared__ int c_pos[2][NW], c_row[2][NW];
   __shared__ double c_val[2][NW][NB + 2];
@@ -597,6 +597,11 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
       // does not write lands in the factor store with the first trailing update
       const double *S = k0 == 0 ? A : W;
       const double *Sb = k0 == 0 ? b : wb;
+      // The panel and the U12 phase are chains of short dependent steps, one barrier apart;
+      // the trailing update is throughput work.  Two workgroups share a CU, so while one runs
+      // its trailing update the other's panel waves take the issue slots first.
+      __builtin_amdgcn_s_setprio (2);
+
       // ---- 1. panel factorization in registers: thread t holds panel rows t + BS pr ----
       // Each step publishes, per wave, the best candidate (GSL key, position) together with
       // that row's values, so after one barrier every thread can eliminate against the winner.
@@ -799,61 +804,86 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
       __syncthreads ();
 
       // ---- 3. A22 -= L21 U12 (only when the panel is full) ----
-      // Wave tiles of 32 x 32 elements: lane (rg, cg) = (lane >> 3, lane & 7) updates the 4 x 4
-      // elements at rows i0 + 8 r and columns c0 + 8 c (i0 = tile row + rg, c0 = tile column +
-      // cg, r, c < 4).  Per update step a lane reads 4 multipliers and 4 U12 values for 16
-      // updates: a quarter of the LDS bytes per update of a tile with one column per lane,
+      // Wave tiles of 8 TR x 32 elements: lane (rg, cg) = (lane >> 3, lane & 7) updates the
+      // TR x 4 elements at rows i0 + 8 r and columns c0 + 8 c (i0 = tile row + rg, c0 = tile
+      // column + cg).  Per update step a lane reads TR multipliers and 4 U12 values for 4 TR
+      // updates (TR = 8 with 256 threads, whose waves have the registers; 4 with 512): a
+      // quarter to a third of the LDS bytes per update of a tile with one column per lane,
       // whose multipliers are whole-wave broadcasts (the LDS, not the FP64 pipes, bounded that
       // one).  Each element takes its NB updates in ascending kk with the product rounded
       // first.  The next tile's A22 values are read before the current tile is updated; rows
       // and columns past n are clamped to n-1 for the reads and not stored.
+      __builtin_amdgcn_s_setprio (0);
       const int nr = n - k1;
       if (nr > 0)
       {
-        constexpr int T = 4;
-        const int nt = (nr + 31) >> 5;
-        const int nit = nt * nt;
+        constexpr int TR = 4, TC = 4;
+        const int nrt = (nr + 8 * TR - 1) / (8 * TR);
+        const int nct = (nr + 31) >> 5;
+        const int nit = nrt * nct;
         const int rg = lane >> 3, cg = lane & 7;
-        double an[T][T];
+        double an[TR][TC];
         int it = wv;
         if (it < nit)
-          blk_load_a22<T> (an, A, W, phys, k0, n, k1 + (it / nt) * 32 + rg, k1 + (it % nt) * 32 + cg);
+          blk_load_a22<TR, TC> (an, A, W, phys, k0, n, k1 + (it / nct) * 8 * TR + rg, k1 + (it % nct) * 32 + cg);
         for (; it < nit; it += NW)
         {
-          const int i0 = k1 + (it / nt) * 32 + rg;
-          const int c0 = k1 + (it % nt) * 32 + cg;
-          double a[T][T];
+          const int i0 = k1 + (it / nct) * 8 * TR + rg;
+          const int c0 = k1 + (it % nct) * 32 + cg;
+          double a[TR][TC];
 #pragma unroll
-          for (int r = 0; r < T; r++)
+          for (int r = 0; r < TR; r++)
 #pragma unroll
-            for (int c = 0; c < T; c++)
+            for (int c = 0; c < TC; c++)
               a[r][c] = an[r][c];
           const int nx = it + NW;
           if (nx < nit)
-            blk_load_a22<T> (an, A, W, phys, k0, n, k1 + (nx / nt) * 32 + rg, k1 + (nx % nt) * 32 + cg);
-          int pl[T];                   // LDS offsets of the tile rows' panel rows
-          unsigned cc[T];              // the lane's columns (clamped)
+            blk_load_a22<TR, TC> (an, A, W, phys, k0, n, k1 + (nx / nct) * 8 * TR + rg, k1 + (nx % nct) * 32 + cg);
+          int pl[TR];                  // LDS offsets of the tile rows' panel rows
+          unsigned cc[TC];             // the lane's columns (clamped)
 #pragma unroll
-          for (int r = 0; r < T; r++)
+          for (int r = 0; r < TR; r++)
             pl[r] = phys[min (i0 + 8 * r, n - 1) - k0] * PP;
 #pragma unroll
-          for (int c = 0; c < T; c++)
+          for (int c = 0; c < TC; c++)
             cc[c] = (unsigned) min (c0 + 8 * c, n - 1);
 #pragma unroll
           for (int kk = 0; kk < NB; kk++)
           {
-            double l[T], u[T];
+            double l[TR], u[TC];
 #pragma unroll
-            for (int r = 0; r < T; r++)
+            for (int r = 0; r < TR; r++)
               l[r] = P[pl[r] + kk];
 #pragma unroll
-            for (int c = 0; c < T; c++)
+            for (int c = 0; c < TC; c++)
               u[c] = US ? Us[kk * n + cc[c]] : wrow (W, n, k0 + kk)[cc[c]];
+            if constexpr (BS == 256)
+            {
+              // every product formed before the first subtraction (these waves have the
+              // registers): the subtractions then never wait on a multiply just issued
+              double pr[TR][TC];
 #pragma unroll
-            for (int r = 0; r < T; r++)
+              for (int r = 0; r < TR; r++)
 #pragma unroll
-              for (int c = 0; c < T; c++)
-                a[r][c] = a[r][c] - l[r] * u[c];
+                for (int c = 0; c < TC; c++)
+                {
+                  pr[r][c] = l[r] * u[c];
+                  __asm__ volatile ("" : "+v"(pr[r][c]));
+                }
+#pragma unroll
+              for (int r = 0; r < TR; r++)
+#pragma unroll
+                for (int c = 0; c < TC; c++)
+                  a[r][c] = a[r][c] - pr[r][c];
+            }
+            else
+            {
+#pragma unroll
+              for (int r = 0; r < TR; r++)
+#pragma unroll
+                for (int c = 0; c < TC; c++)
+                  a[r][c] = a[r][c] - l[r] * u[c];
+            }
             // one step's operands live at a time (hoisting every step's LDS reads to the top
             // of the unrolled loop would need NB times the registers)
             __builtin_amdgcn_sched_barrier (0);
@@ -862,18 +892,18 @@ __global__ __launch_bounds__ (BS, 4) void k_blocked_f64 (SolveArgs s)
           // element's update chain into its conditional store and reads every step's operands
           // first, far more than the registers hold
 #pragma unroll
-          for (int r = 0; r < T; r++)
+          for (int r = 0; r < TR; r++)
 #pragma unroll
-            for (int c = 0; c < T; c++)
+            for (int c = 0; c < TC; c++)
               __asm__ volatile ("" : "+v"(a[r][c]));
 #pragma unroll
-          for (int r = 0; r < T; r++)
+          for (int r = 0; r < TR; r++)
           {
             const int i = i0 + 8 * r;
             if (i < n)
             {
 #pragma unroll
-              for (int c = 0; c < T; c++)
+              for (int c = 0; c < TC; c++)
                 if (c0 + 8 * c < n)
                   W[(size_t) i * n + (unsigned) (c0 + 8 * c)] = a[r][c];
             }
@@ -1120,7 +1150,9 @@ hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, bool keep
   if (nmax > 64 * BLK_MAXM || !a.F || !a.Y)
     return hipErrorInvalidValue;
   const size_t sm = blocked_f64_smem (nmax);
-  if (nmax <= 512)
+  if (nmax <= 256)
+    launch_blocked_m<256, 16, true> (a, grid, sm, keep, st);
+  else if (nmax <= 512)
     launch_blocked_m<BLK_BS, 16, true> (a, grid, sm, keep, st);
   else
     launch_blocked_m<BLK_BS, 8, false> (a, grid, sm, keep, st);
@@ -1147,7 +1179,7 @@ void fp64_kernel_symbol (int nmax, bool keep, char *buf, size_t len)
     snprintf (buf, len, "%s + k_bsub_f64<%d>", t, nmax <= 128 ? 2 : 3);
   }
   else if (nmax <= 512)
-    snprintf (buf, len, "k_blocked_f64<%d, 16, true, %s> + k_bsub_f64<%d>", BLK_BS, k,
+    snprintf (buf, len, "k_blocked_f64<%d, 16, true, %s> + k_bsub_f64<%d>", nmax <= 256 ? 256 : BLK_BS, k,
               nmax <= 128 ? 2 : nmax <= 192 ? 3 : nmax <= 256 ? 4 : 8);
   else
     snprintf (buf, len, "k_blocked_f64<%d, 8, false, %s> + k_bsub_f64<%d>", BLK_BS, k, BLK_MAXM);
@@ -1174,6 +1206,8 @@ hipError_t setup_solve_f64_attrs (size_t max_smem)
   set_smem_attr (k_wave_f64<16, 4, true>, max_smem);
   set_smem_attr (k_wave_f64<32, 4, true>, max_smem);
   set_smem_attr (k_wave_f64<64, 2, true>, max_smem);
+  set_smem_attr (k_blocked_f64<256, 16, true, false>, max_smem);
+  set_smem_attr (k_blocked_f64<256, 16, true, true>, max_smem);
   set_smem_attr (k_blocked_f64<BLK_BS, 16, true, false>, max_smem);
   set_smem_attr (k_blocked_f64<BLK_BS, 16, true, true>, max_smem);
   set_smem_attr (k_blocked_f64<BLK_BS, 8, false, false>, max_smem);
