@@ -133,6 +133,11 @@ __global__ __launch_bounds__ (64 * NW, 4) void k_tile_f64 (SolveArgs s)
         {
           const int buf = k & 1;
           __syncthreads ();
+          // the wave that owns column k+1 carries the step-to-step chain (its column, then the
+          // next pivot): its instructions go first on its SIMD
+          const int wn = (kc + 1 < CPW) ? kb : kb + 1;  // owner of column k+1
+          if (wv == wn)
+            __builtin_amdgcn_s_setprio (2);
           const int p = __builtin_amdgcn_readfirstlane (s_pv[buf][0]);
           const int pq = __builtin_amdgcn_readfirstlane (s_pv[buf][1]);
           singular |= s_pv[buf][2];
@@ -195,7 +200,6 @@ __global__ __launch_bounds__ (64 * NW, 4) void k_tile_f64 (SolveArgs s)
 
             // ---- step k+1 looks ahead: its column first, then its pivot ----
             const int kn = (kc + 1 < CPW) ? kc + 1 : 0;   // column k+1 in its owner's block
-            const int wn = (kc + 1 < CPW) ? kb : kb + 1;  // owner of column k+1
             if (wv == wn && k + 1 < n)
             {
 #pragma unroll
@@ -205,6 +209,8 @@ __global__ __launch_bounds__ (64 * NW, 4) void k_tile_f64 (SolveArgs s)
               tile_pivot<R, CPW, KEEP> (a, pos, kn, k + 1, n, lane, s_l[buf ^ 1], s_pv[buf ^ 1], s_pq);
               u[kn] = 0.0;                   // column k+1 holds its multipliers now
             }
+            if (wv == wn)
+              __builtin_amdgcn_s_setprio (0);
 
             // ---- update: rows below the pivot (exec-masked per register slot, so finished rows
             //      are not touched) ----
