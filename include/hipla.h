@@ -109,6 +109,17 @@ int hipla_invert (hipla_plan plan, const double *dA, double *dInv, int *dcodes, 
 int hipla_adjoint (hipla_plan plan, const double *dX, const double *dG, double *dLambda, double *dGradA,
                    void *stream);
 
+/* Macro-atom emissivities without forming N^-1 (needs HIPLA_FLAG_KEEP_FACTORS, HIPLA_MODE_FP64).
+ * Replaces invert_matrix + B = N^-1 R (source/macro_accelerate.c:315-334) + the contraction of
+ * source/macro_gen_f.c:490-506 by one factorisation and one transposed solve per cell: with N the
+ * macro-atom matrix (A's layout, order nlevels_macro + 1, the k-packet last) and, in B's layout,
+ * rdiag = diag(R), abs = (matom_abs[0..], kpkt_abs), emit = (level_emit[0..], kpkt_emit):
+ *   out_j = (N^-T abs)_j * rdiag_j * emit_j   (matom_emiss for j < n-1, kpkt_emiss for j = n-1).
+ * dcodes gets solve_matrix's codes for N x = abs; a cell with code 4 (singular N) is not
+ * written.  Equal to the reference's numbers up to rounding (not in its summation order). */
+int hipla_macro_emiss (hipla_plan plan, const double *dN, const double *dRdiag, const double *dAbs,
+                       const double *dEmit, double *dOut, int *dcodes, void *stream);
+
 /* Time the kernel variants for this plan's size histogram and keep the fastest per size class
  * (the choice is cached per process and, if HIPLA_TUNE_CACHE names a file, across processes).
  * Runs synchronously on `stream`.  dA..dX must be valid inputs as for hipla_solve. */

@@ -116,6 +116,26 @@ def invert_matrix(m):
     return code, (inv if code == 0 else None)
 
 
+def macro_emiss(N, rdiag, absorbed, emit):
+    """The reference's macro-atom emissivities of one cell, in its order: (code, out or None).
+
+    N^-1 by the GSL-order inverse (``invert_matrix``, source/macro_accelerate.c:315), then
+    B_ij = N^-1_ij * R_jj (:324-334), then for each absorbing level i in ascending order
+    emiss_j += a_i * B_ij over every j (source/macro_gen_f.c:490-499, the k-packet row last,
+    :503-506), and finally emiss_j *= emit_j (:504, :507).  ``absorbed`` is
+    (matom_abs[0..n-2], kpkt_abs), ``emit`` (level_emit[0..n-2], kpkt_emit); out[n-1] is
+    kpkt_emiss."""
+    code, inv = invert_matrix(N)
+    if code != 0:
+        return code, None
+    rdiag, a, e = _c(rdiag), _c(absorbed), _c(emit)
+    B = inv * rdiag[None, :]
+    out = np.zeros(N.shape[0])
+    for i in range(N.shape[0]):
+        out += a[i] * B[i, :]
+    return 0, out * e
+
+
 def lu_solve_transpose(lu, perm, g):
     lu = _c(lu)
     n = lu.shape[0]

@@ -85,6 +85,7 @@ struct hipla_plan_s
   double *d_F = nullptr;      // factor store of the workgroup kernels (d_LU when factors are kept)
   bool own_F = false;
   double *d_Y = nullptr;      // forward-substituted right-hand sides (B's layout)
+  double *d_lam = nullptr;    // keep-factors: transposed-solve results of hipla_macro_emiss (B's layout)
   // the plan's own stream (graph capture, tuning)
   std::vector<hipStream_t> aux;
   // graph cache

@@ -78,7 +78,7 @@ static int free_plan_device (hipla_plan p)
   p->gexec = nullptr;
   p->graph = nullptr;
   void *bufs[] = { p->d_n,   p->d_offA, p->d_offV, p->d_list, p->d_list_desc, p->d_inv_work, p->d_fallback,
-                   p->d_LU,  p->d_piv,  p->own_F ? p->d_F : nullptr, p->d_Y };
+                   p->d_LU,  p->d_piv,  p->own_F ? p->d_F : nullptr, p->d_Y, p->d_lam };
   for (void *b : bufs)
     if (b)
       (void) hipFree (b);
@@ -223,6 +223,7 @@ extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, con
   {
     PCHECK (hipMalloc (&p->d_LU, sizeof (double) * p->totA));
     PCHECK (hipMalloc (&p->d_piv, sizeof (int) * p->totV));
+    PCHECK (hipMalloc (&p->d_lam, sizeof (double) * p->totV));
   }
   // factor and y stores of the workgroup factor kernels (cells above 64), read by the
   // back-substitution kernel; kept factors double as the factor store

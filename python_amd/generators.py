@@ -155,6 +155,16 @@ def macro_q(rng, ncells: int, n: int, density=(0.10, 0.40), val_range=(1e-8, 1.0
     return N
 
 
+def macro_emiss_vectors(rng, N):
+    """The per-cell vectors of the macro-atom emissivity contraction for one N = I - Q
+    (source/macro_accelerate.c:253-298, source/macro_gen_f.c:482-506): diag(R) with N's row sums
+    (the normalisation the reference checks, macro_accelerate.c:286-298: R_ii - sum_j N_ij = 0),
+    absorbed energies (matom_abs, kpkt_abs last) spread over six decades, emission fractions in
+    [0, 1)."""
+    n = N.shape[0]
+    return N.sum(axis=1), _logu(rng, 1e-6, 1.0, n), rng.random(n)
+
+
 @dataclass
 class RaggedBatch:
     """A ragged batch of square FP64 cells packed row-major back to back."""

@@ -178,6 +178,26 @@ class Plan:
               "hipla_adjoint")
         return Lam, GradA
 
+    def macro_emiss(self, N, rdiag, absorbed, emit, out=None, codes=None, stream=None):
+        """Macro-atom emissivities out_j = (N^-T a)_j R_jj e_j per cell, N^-1 never formed
+        (hipla_macro_emiss; the reference's invert_matrix + B = N^-1 R + a^T B contraction,
+        source/macro_accelerate.c:315-334, source/macro_gen_f.c:490-506).  N has A's layout;
+        rdiag, absorbed, emit and out have B's.  The plan needs keep_factors=True, mode "fp64".
+        Returns (out, codes); a cell with code 4 (singular N) is not written."""
+        import torch
+        self._check_tensor(N, self.totA, torch.float64, "N")
+        for t, name in ((rdiag, "rdiag"), (absorbed, "absorbed"), (emit, "emit")):
+            self._check_tensor(t, self.totV, torch.float64, name)
+        if out is None:
+            out = torch.zeros_like(absorbed)
+        if codes is None:
+            codes = torch.empty(self.ncells, dtype=torch.int32, device=absorbed.device)
+        self._check_tensor(out, self.totV, torch.float64, "out")
+        self._check_tensor(codes, self.ncells, torch.int32, "codes")
+        check(lib().hipla_macro_emiss(self._h, _ptr(N), _ptr(rdiag), _ptr(absorbed), _ptr(emit), _ptr(out),
+                                      _ptr(codes), _stream_handle(stream)), "hipla_macro_emiss")
+        return out, codes
+
     def time_groups(self, A, B, X, codes, stats=None, reps: int = 5, classes_only: bool = False):
         """Per-group average ms (HIP events on the plan's stream): (groups, fallback pass, whole),
         or the groups alone with ``classes_only`` (nothing else is launched then)."""

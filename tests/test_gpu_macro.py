@@ -1,0 +1,54 @@
+"""GPU check of hipla_macro_emiss (macro-atom emissivities from one factorisation and one
+transposed solve, N^-1 never formed) against the oracle's restatement of the reference's order
+(invert_matrix, B = N^-1 R, a^T B; source/macro_accelerate.c:315-334,
+source/macro_gen_f.c:490-506).
+
+Tolerance: per-cell normwise relative error <= 1e-10 (the two computations differ by rounding
+only: the reference's summation order is not kept).  A singular N gets code 4 and its output is
+not written.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def test_macro_emiss_matches_oracle(hip, oracle_lib):
+    import torch
+    from python_amd.generators import macro_emiss_vectors, macro_q, pack
+    rng = np.random.default_rng(11)
+    sizes = (1, 5, 12, 40, 64, 65, 76, 100, 129, 200, 256, 300)
+    mats, rv, av, ev = [], [], [], []
+    for n in sizes:
+        N = macro_q(rng, 1, n)[0]
+        r, a, e = macro_emiss_vectors(rng, N)
+        mats.append(N), rv.append(r), av.append(a), ev.append(e)
+    sing = np.eye(20)
+    sing[7, :] = 0.0                                  # a level with no way out: N singular
+    mats.append(sing), rv.append(np.ones(20)), av.append(np.ones(20)), ev.append(np.ones(20))
+    b = pack(mats, av)
+    plan = hip.Plan.for_batch(b, mode="fp64", graph=False, keep_factors=True)
+    Nd, Ad = hip.to_device(b)
+    Rd = torch.from_numpy(np.concatenate(rv)).cuda()
+    Ed = torch.from_numpy(np.concatenate(ev)).cuda()
+    out = torch.full_like(Ad, -7.0)
+    for rep in range(2):                              # the second call reuses the plan's buffers
+        out.fill_(-7.0)
+        out, codes = plan.macro_emiss(Nd, Rd, Ad, Ed, out=out)
+        torch.cuda.synchronize()
+        oh, ch = out.cpu().numpy(), codes.cpu().numpy()
+        for c, n in enumerate(b.n):
+            o = b.offV[c]
+            got = oh[o:o + n]
+            if c == len(sizes):
+                assert ch[c] == 4, rep
+                assert np.all(got == -7.0), rep
+                continue
+            assert ch[c] == 0, (rep, n, ch[c])
+            code, want = oracle_lib.macro_emiss(mats[c], rv[c], av[c], ev[c])
+            assert code == 0
+            err = np.abs(got - want).max() / np.abs(want).max()
+            print(rep, n, err, flush=True)
+            assert err <= TOL, (rep, n, err)
