@@ -120,6 +120,20 @@ int hipla_adjoint (hipla_plan plan, const double *dX, const double *dG, double *
 int hipla_macro_emiss (hipla_plan plan, const double *dN, const double *dRdiag, const double *dAbs,
                        const double *dEmit, double *dOut, int *dcodes, void *stream);
 
+/* Binary ragged-batch files (host only; no counterpart in the reference, whose tests exchange
+ * single matrices as 7-digit "%e" text, source/tests/test_matrix.c:52-62).  A batch is kept
+ * bit-exact in the layout the plans take: a 64-byte header ("HIPLARB1", version 1, flags bit 0 =
+ * B present, ncells, totA, totV), int32 n[ncells] padded to 8 bytes, int64 offA[ncells],
+ * int64 offV[ncells], FP64 A[totA], then FP64 B[totV] if present; little-endian.
+ * hipla_batch_write stores a batch (B may be NULL); hipla_batch_info reads the sizes (has_b may
+ * be NULL); hipla_batch_read fills caller arrays of those sizes (B may be NULL to skip it).
+ * Every cell's extent and the file size are checked.  0, or 5 with the reason in
+ * hipla_last_error_detail. */
+int hipla_batch_write (const char *path, int ncells, const int *n, const int64_t *offA, const int64_t *offV,
+                       const double *A, const double *B);
+int hipla_batch_info (const char *path, int *ncells, int64_t *totA, int64_t *totV, int *has_b);
+int hipla_batch_read (const char *path, int *n, int64_t *offA, int64_t *offV, double *A, double *B);
+
 /* Time the kernel variants for this plan's size histogram and keep the fastest per size class
  * (the choice is cached per process and, if HIPLA_TUNE_CACHE names a file, across processes).
  * Runs synchronously on `stream`.  dA..dX must be valid inputs as for hipla_solve. */

@@ -49,6 +49,9 @@ SIGNATURES = {
     "hipla_invert": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "hipla_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "hipla_macro_emiss": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "hipla_batch_write": (c_int, [c_char_p, c_int, _ip, _i64p, _i64p, _dp, _dp]),
+    "hipla_batch_info": (c_int, [c_char_p, _ip, _i64p, _i64p, _ip]),
+    "hipla_batch_read": (c_int, [c_char_p, _ip, _i64p, _i64p, _dp, _dp]),
     "hipla_plan_autotune": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "hipla_plan_describe": (c_int, [c_void_p, ctypes.c_char_p, c_int]),
     "hipla_plan_time_groups": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, _dp, c_int]),
