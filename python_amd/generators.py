@@ -11,6 +11,9 @@ generators restate their *structure* with closed-form random rates:
   (ones over that element's ions, b = abundance), as ``populate_ion_rate_matrix`` does
   (matrix_ion.c:633-655).  b is zero except on those rows.
 * Macro-atom inverse (``macro_accelerate.c:269-284``): N = I - Q with Q substochastic.
+* Macro-atom level populations (``macro_gov.c:366-440``, ``macro_pops_fill_rate_matrix``
+  ``:535-749``): per element a rate matrix over its levels with M[to, from] += rate and
+  M[from, from] -= rate for every transition, row 0 replaced by ones, b = e_0.
 
 All randomness comes from ``numpy.random.default_rng(seed)`` so the same seed gives the same
 bytes on any machine.  Layout of a batch: cell c is the row-major n_c x n_c matrix at
@@ -153,6 +156,32 @@ def macro_q(rng, ncells: int, n: int, density=(0.10, 0.40), val_range=(1e-8, 1.0
         N[c] = -Q
         N[c][np.arange(n), np.arange(n)] = 1.0
     return N
+
+
+def macro_pops_matrices(rng, ncells: int, n: int, rate_range=(1e-6, 1e8), density=0.15):
+    """ncells macro-atom level-population systems of order n: returns (A [ncells,n,n], b [ncells,n]).
+
+    Every transition from -> to adds its rate at M[to, from] and takes it from M[from, from]
+    (source/macro_gov.c:593-597, 628-629, 661-662, 693-708, 737-738), so every column sums to
+    zero; adjacent levels are always linked both ways (bound-bound / bound-free between
+    neighbouring levels and stages) and random further pairs are linked at ``density``.  Row 0
+    is then replaced by the normalisation (all ones) and b = e_0 (macro_gov.c:405-408, 436)."""
+    A = np.zeros((ncells, n, n))
+    b = np.zeros((ncells, n))
+    b[:, 0] = 1.0
+    for c in range(ncells):
+        M = np.zeros((n, n))
+        link = rng.random((n, n)) < density
+        np.fill_diagonal(link, False)
+        idx = np.arange(n - 1)
+        link[idx, idx + 1] = link[idx + 1, idx] = True
+        frm, to = np.nonzero(link)
+        rates = _logu(rng, *rate_range, len(frm))
+        np.add.at(M, (to, frm), rates)
+        np.add.at(M, (frm, frm), -rates)
+        M[0, :] = 1.0
+        A[c] = M
+    return A, b
 
 
 def macro_emiss_vectors(rng, N):

@@ -58,3 +58,20 @@ def test_te_derivative_matches_finite_difference():
     fd = (Ap - Am) / (2 * h)[:, None, None]
     scale = np.abs(d["dA"]).max(axis=(1, 2))[:, None, None] + 1e-300
     assert np.all(np.abs(fd - d["dA"]) <= 1e-5 * scale)
+
+
+def test_macro_pops_structure():
+    """Level-population systems (source/macro_gov.c:366-440): below row 0 every column of the
+    rate matrix keeps only the transitions into other levels, so columns sum to minus the
+    removed row-0 entries' rates; the oracle's populations are positive and sum to 1."""
+    import oracle
+    from python_amd.generators import macro_pops_matrices
+    A, b = macro_pops_matrices(np.random.default_rng(4), 4, 15)
+    for M, rhs in zip(A, b):
+        assert np.all(M[0] == 1.0) and rhs[0] == 1.0 and not rhs[1:].any()
+        offd = M[1:] - np.diag(np.diag(M))[1:]
+        assert np.all(offd >= 0.0)
+        assert np.all(np.diag(M)[1:] < 0.0)
+        code, x = oracle.solve_matrix(M, rhs)
+        assert code == 0
+        assert np.all(x > 0.0) and abs(x.sum() - 1.0) <= 1e-12

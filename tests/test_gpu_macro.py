@@ -52,3 +52,24 @@ def test_macro_emiss_matches_oracle(hip, oracle_lib):
             err = np.abs(got - want).max() / np.abs(want).max()
             print(rep, n, err, flush=True)
             assert err <= TOL, (rep, n, err)
+
+
+def test_macro_pops_batched_equal_oracle(hip, oracle_lib):
+    """Per-element macro-atom level-population systems (source/macro_gov.c:366-440), many cells
+    and elements in one batched plan: values equal to the oracle's GSL-order solve, codes equal,
+    populations summing to 1."""
+    from python_amd.generators import macro_pops_matrices, pack
+    rng = np.random.default_rng(8)
+    mats, rhs = [], []
+    for n, count in ((5, 60), (12, 40), (20, 30), (40, 20), (76, 10), (130, 4)):
+        A, b = macro_pops_matrices(rng, count, n)
+        mats += list(A)
+        rhs += list(b)
+    batch = pack(mats, rhs)
+    X, codes, _ = hip.solve_batch(batch, mode="fp64")
+    Xo, co = oracle_lib.solve_batch(batch.n, batch.offA, batch.offV, batch.A, batch.B, nthreads=8)
+    assert np.array_equal(codes, co)
+    assert np.array_equal(X, Xo)
+    for c in range(batch.ncells):
+        o, n = batch.offV[c], batch.n[c]
+        assert abs(X[o:o + n].sum() - 1.0) <= 1e-10
