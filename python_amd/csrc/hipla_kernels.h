@@ -19,7 +19,10 @@ hipError_t launch_wave_f64 (const SolveArgs &a, int nc, bool keep, int grid, hip
 // workgroup-resident route (64 < n <= wgr_f64_limit ()): register-tiled factorization
 // (k_tile.hip) + back substitution
 hipError_t launch_wgr_f64 (const SolveArgs &a, int nmax, bool keep, int grid, hipStream_t st);
-hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, bool keep, hipStream_t st);
+hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, bool keep, hipStream_t st,
+                               bool with_bsub = true);
+// back substitution + residual check of cells a workgroup factor kernel has factored
+hipError_t launch_bsub_cells (const SolveArgs &a, int nmax, hipStream_t st);
 int blocked_f64_wgs_per_cu (int nmax);   // resident workgroups per CU (persistent grid size)
 hipError_t setup_solve_f64_attrs (size_t max_smem);
 int wgr_f64_limit ();   // largest n of the workgroup-resident FP64 route

@@ -88,6 +88,10 @@ struct hipla_plan_s
   double *d_lam = nullptr;    // keep-factors: transposed-solve results of hipla_macro_emiss (B's layout)
   // the plan's own stream (graph capture, tuning)
   std::vector<hipStream_t> aux;
+  // a second stream for the back substitution of the largest class, forked after its factor
+  // kernel and joined at the end of the solve (default priority like every stream here)
+  hipStream_t fork_st = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // graph cache
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;

@@ -85,6 +85,14 @@ static int free_plan_device (hipla_plan p)
   for (auto s : p->aux)
     (void) hipStreamDestroy (s);
   p->aux.clear ();
+  if (p->fork_st)
+    (void) hipStreamDestroy (p->fork_st);
+  if (p->ev_fork)
+    (void) hipEventDestroy (p->ev_fork);
+  if (p->ev_join)
+    (void) hipEventDestroy (p->ev_join);
+  p->fork_st = nullptr;
+  p->ev_fork = p->ev_join = nullptr;
   return 0;
 }
 
@@ -245,6 +253,9 @@ extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, con
   }
   p->aux.resize (1, nullptr);
   PCHECK (hipStreamCreateWithFlags (&p->aux[0], hipStreamNonBlocking));
+  PCHECK (hipStreamCreateWithFlags (&p->fork_st, hipStreamNonBlocking));
+  PCHECK (hipEventCreateWithFlags (&p->ev_fork, hipEventDisableTiming));
+  PCHECK (hipEventCreateWithFlags (&p->ev_join, hipEventDisableTiming));
 #undef PCHECK
   *out = p;
   return 0;
@@ -357,9 +368,25 @@ int plan_enqueue_solve (hipla_plan p, const double *dA, const double *dB, double
   const bool per_cell = (p->flags & HIPLA_FLAG_PER_CELL) != 0;
   // the size classes one after another on the caller's stream: each class's launch already
   // spreads over every CU, and classes run side by side only contend for LDS and registers
-  // (measured slower than back to back)
-  for (const Group &g : p->groups)
+  // (measured slower than back to back).  The exception: a last class on the blocked kernel
+  // goes first, and its back substitution (one latency-bound wave per cell) runs on the fork
+  // stream beside the smaller classes; the caller's stream waits for it at the end.
+  const int ng = (int) p->groups.size ();
+  const bool fork = !per_cell && ng > 1 && p->groups.back ().kind == KK_BLOCKED_F64;
+  if (fork)
   {
+    const Group &g = p->groups.back ();
+    SolveArgs a = make_args (p, dA, dB, dX, dcodes, g);
+    const bool keep = (p->flags & HIPLA_FLAG_KEEP_FACTORS) != 0;
+    HIPLA_CHECK (launch_blocked_f64 (a, g.nmax, std::min (g.grid, std::max (a.nlist, 1)), keep, st, false));
+    HIPLA_CHECK (hipEventRecord (p->ev_fork, st));
+    HIPLA_CHECK (hipStreamWaitEvent (p->fork_st, p->ev_fork, 0));
+    HIPLA_CHECK (launch_bsub_cells (a, g.nmax, p->fork_st));
+    HIPLA_CHECK (hipEventRecord (p->ev_join, p->fork_st));
+  }
+  for (int gi = 0; gi < ng - (fork ? 1 : 0); gi++)
+  {
+    const Group &g = p->groups[gi];
     SolveArgs a = make_args (p, dA, dB, dX, dcodes, g);
     if (per_cell)
     {
@@ -375,6 +402,8 @@ int plan_enqueue_solve (hipla_plan p, const double *dA, const double *dB, double
     else
       HIPLA_CHECK (launch_group_f64 (p, a, g, st));
   }
+  if (fork)
+    HIPLA_CHECK (hipStreamWaitEvent (st, p->ev_join, 0));
   return 0;
 }
 

@@ -1031,6 +1031,57 @@ int plan_enqueue_mixed (hipla_plan p, const double *dA, const double *dB, double
   // same FP64 kernel (their cells are contiguous in the plan's list) are solved by one launch, so
   // a persistent grid covers all of them at once.
   const DeviceInfo &dev = hipla_device_info ();
+  if (!any_mixed)
+  {
+    // every group straight to FP64: merge, then run the last unit first when it is a blocked
+    // one.  Its factor kernel fills the chip; its back substitution (one latency-bound wave per
+    // cell, a fraction of each SIMD) then runs on the fork stream beside the smaller classes,
+    // and the caller's stream waits for it at the end.
+    std::vector<Group> units;
+    for (int gi = 0; gi < nb;)
+    {
+      Group u = p->groups[gi];
+      int gj = gi + 1;
+      for (; gj < nb; gj++)
+      {
+        const Group &h = p->groups[gj];
+        if (fp64_family (h.nmax) != fp64_family (u.nmax) || h.first != u.first + u.count)
+          break;
+        u.count += h.count;
+        u.nmax = std::max (u.nmax, h.nmax);
+      }
+      u.grid = std::min (u.count, blocked_f64_wgs_per_cu (u.nmax) * dev.num_cu);
+      units.push_back (u);
+      gi = gj;
+    }
+    const bool fork = units.size () > 1 && units.back ().nmax > wgr_f64_limit ();
+    if (fork)
+    {
+      const Group &u = units.back ();
+      SolveArgs a = group_args (p, dA, dB, dX, dcodes, u);
+      HIPLA_CHECK (launch_blocked_f64 (a, u.nmax, std::max (1, u.grid), false, st, false));
+      if (dstats)
+      {
+        hipLaunchKernelGGL (k_mark_fp64_stats, dim3 ((u.count + 255) / 256), dim3 (256), 0, st, a.list, u.count,
+                            dstats);
+        HIPLA_CHECK (hipGetLastError ());
+      }
+      HIPLA_CHECK (hipEventRecord (p->ev_fork, st));
+      HIPLA_CHECK (hipStreamWaitEvent (p->fork_st, p->ev_fork, 0));
+      HIPLA_CHECK (launch_bsub_cells (a, u.nmax, p->fork_st));
+      HIPLA_CHECK (hipEventRecord (p->ev_join, p->fork_st));
+      units.pop_back ();
+    }
+    for (const Group &u : units)
+    {
+      SolveArgs a = group_args (p, dA, dB, dX, dcodes, u);
+      if ((rc = launch_group_direct_fp64 (p, a, u, dstats, st)))
+        return rc;
+    }
+    if (fork)
+      HIPLA_CHECK (hipStreamWaitEvent (st, p->ev_join, 0));
+    return 0;
+  }
   for (int gi = 0; gi < nb;)
   {
     const Group &g = p->groups[gi];

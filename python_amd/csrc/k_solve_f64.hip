@@ -1143,7 +1143,14 @@ static void launch_blocked_m (const SolveArgs &a, int grid, size_t sm, bool keep
     hipLaunchKernelGGL ((k_blocked_f64<BS, NB, US, false>), dim3 (grid), dim3 (BS), sm, st, a);
 }
 
-hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, bool keep, hipStream_t st)
+hipError_t launch_bsub_cells (const SolveArgs &a, int nmax, hipStream_t st)
+{
+  if (a.nlist <= 0)
+    return hipSuccess;
+  return launch_bsub_f64 (a, nmax, st);
+}
+
+hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, bool keep, hipStream_t st, bool with_bsub)
 {
   if (a.nlist <= 0)
     return hipSuccess;
@@ -1157,7 +1164,7 @@ hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, bool keep
   else
     launch_blocked_m<BLK_BS, 8, false> (a, grid, sm, keep, st);
   hipError_t e = hipGetLastError ();
-  if (e != hipSuccess)
+  if (e != hipSuccess || !with_bsub)
     return e;
   return launch_bsub_f64 (a, nmax, st);
 }
