@@ -122,3 +122,42 @@ def test_mixed_autotuned_fp64_classes_merged(hip, oracle_lib):
             for c in np.where((batch.n == g["nmax"]) & (co != 4))[0]:
                 o, n = batch.offV[c], batch.n[c]
                 assert np.array_equal(Xh[o:o + n], Xo[o:o + n]), (rep, c)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_forked_back_substitution_graph_replays(hip, oracle_lib, mode):
+    """A plan whose largest class runs on the blocked kernel and has smaller classes beside it:
+    the largest class's back substitution runs on the plan's second stream (forked after its
+    factor kernel, joined at the end), in eager calls and in graph replays.  Every replay equals
+    the oracle (FP64 kernels: bit for bit) with codes equal, with eager work and synchronisations
+    between the replays."""
+    import torch
+    from python_amd.generators import ion_matrices, pack
+    rng = np.random.default_rng(21)
+    mats, rhs = [], []
+    for n, count in ((12, 40), (48, 30), (100, 20), (230, 12)):
+        A, b = ion_matrices(rng, count, n)
+        mats += list(A)
+        rhs += list(b)
+    batch = pack(mats, rhs)
+    Xo, co = oracle_lib.solve_batch(batch.n, batch.offA, batch.offV, batch.A, batch.B, nthreads=8)
+    A, B = hip.to_device(batch)
+    for graph in (False, True):
+        plan = hip.Plan.for_batch(batch, mode=mode, graph=graph)
+        X = torch.zeros_like(B)
+        codes = torch.empty(batch.ncells, dtype=torch.int32, device=B.device)
+        if mode == "mixed":
+            plan.autotune(A, B, X, codes)
+        for rep in range(3):
+            X.zero_()
+            plan.solve(A, B, X, codes)
+            torch.cuda.synchronize()
+            Xh, ch = X.cpu().numpy(), codes.cpu().numpy()
+            assert np.array_equal(ch, co), (graph, rep)
+            if mode == "fp64" or all(g["direct_fp64"] for g in plan.describe()["groups"]):
+                assert np.array_equal(Xh, Xo), (graph, rep)
+            else:
+                err = _relerr(batch, Xh, Xo, co)
+                assert err.max() <= TOL, (graph, rep, err.max())
+            torch.empty(1 << 20, device=B.device).normal_()      # eager work between replays
+        plan.close()
