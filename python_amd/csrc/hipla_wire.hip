@@ -131,7 +131,10 @@ static int read_header (std::FILE *f, const char *path, Header &h, const char *w
     hipla_set_detail ("%s: %s is not a ragged-batch file", who, path);
     return 5;
   }
-  if (h.version != kVersion || h.ncells < 0 || h.totA < 0 || h.totV < 0 || h.ncells > INT32_MAX)
+  // sizes far past any file (2^56 doubles) are refused before the byte count is formed, so it
+  // cannot overflow
+  if (h.version != kVersion || h.ncells < 0 || h.totA < 0 || h.totV < 0 || h.ncells > INT32_MAX ||
+      h.totA > (int64_t (1) << 56) || h.totV > (int64_t (1) << 56))
   {
     hipla_set_detail ("%s: %s has version %u / bad sizes", who, path, h.version);
     return 5;
