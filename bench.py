@@ -73,11 +73,13 @@ def measure_roofline(plan, A, B, reps=10):
     dk = by_kernel[dom]
     bw = dk["bytes"] / (dk["ms"] * 1e-3) / 1e9
     fl = dk["flops"] / (dk["ms"] * 1e-3) / 1e12
+    # the FP64 kernels run on the vector ALUs (GSL's order: separately rounded products, no
+    # matrix cores), so the compute roof is the FP64 vector peak
     if bw / HBM_PEAK_GBS >= fl / FP64_PEAK_TFLOPS:
         roof = {"bound": "hbm", "achieved": round(bw, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(bw / HBM_PEAK_GBS, 5)}
     else:
-        roof = {"bound": "mfma", "achieved": round(fl, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        roof = {"bound": "fp64", "achieved": round(fl, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(fl / FP64_PEAK_TFLOPS, 5)}
     roof["traffic"] = None
     roof["kernel"] = dom
@@ -117,10 +119,68 @@ def _emit(out):
     return 0
 
 
+def bench_c1(args):
+    """The plain small case: 900 cells of the n = 30 ionization matrix (H, He, C, N, O + slack,
+    block-tridiagonal), seed 1.  The batched FP64 GSL-order solve (one graph launch per step)
+    against one launch per matrix and the CPU oracle on the same host, values and codes checked
+    against the oracle in every cell."""
+    import torch
+    import oracle
+    import python_amd as pa
+    from python_amd.generators import plain_small
+    batch = plain_small()
+    A, B = pa.to_device(batch)
+    plan = pa.Plan.for_batch(batch, mode="fp64", graph=True)
+    X = torch.zeros_like(B)
+    codes = torch.empty(batch.ncells, dtype=torch.int32, device=B.device)
+    ms = _timed_ms(lambda: plan.solve(A, B, X, codes), max(args.steps, 50), args.warmup)
+    per = pa.Plan.for_batch(batch, mode="fp64", graph=False, per_cell=True)
+    Xp = torch.zeros_like(B)
+    ms_per = _timed_ms(lambda: per.solve(A, B, Xp, codes), 3, 1)
+    oracle.build()
+    cores = _cores()
+    Xo, co = oracle.solve_batch(batch.n, batch.offA, batch.offV, batch.A, batch.B, nthreads=cores)
+    plan.solve(A, B, X, codes)
+    torch.cuda.synchronize()
+    equal = bool(np.array_equal(X.cpu().numpy(), Xo) and np.array_equal(codes.cpu().numpy(), co))
+    reps_c, total = 0, 0.0
+    while total < 10.0 and reps_c < 5000:
+        t0 = time.perf_counter()
+        oracle.solve_batch(batch.n, batch.offA, batch.offV, batch.A, batch.B, nthreads=cores)
+        total += time.perf_counter() - t0
+        reps_c += 1
+    reps_1, total_1 = 0, 0.0
+    while total_1 < 5.0 and reps_1 < 2000:
+        t0 = time.perf_counter()
+        oracle.solve_batch(batch.n, batch.offA, batch.offV, batch.A, batch.B, nthreads=1)
+        total_1 += time.perf_counter() - t0
+        reps_1 += 1
+    byts, flops = float(batch.bytes_solve), batch.flops_solve
+    bw = byts / (ms * 1e-3) / 1e9
+    return _emit({
+        "metric": "solves/s, 900 ionization matrices n = 30, FP64 GSL order", "unit": "solves/s",
+        "value": round(batch.ncells / (ms * 1e-3), 1), "ms_per_step": round(ms, 4), "n_gpus": 1, "dtype": "f64",
+        "data": "synthetic (seeded block-tridiagonal ionization matrices, seed 1)",
+        "config": {"workload": "plain_small: 900 cells, n = 30 (H2 He3 C7 N8 O9 + slack)"},
+        "equal_to_oracle_values_and_codes": equal,
+        "plain_fp64_per_matrix": {"solves_per_s": round(batch.ncells / (ms_per * 1e-3), 1),
+                                  "ms_per_step": round(ms_per, 3)},
+        "roofline": {"bound": "hbm", "achieved": round(bw, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(bw / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "k_wave_f64<32, 4, false> (one graph launch)",
+                     "algorithmic_bytes_per_launch": round(byts), "algorithmic_flops_per_launch": round(flops)},
+        "cpu_baseline": {"value": round(reps_c * batch.ncells / total, 1), "unit": "solves/s", "cores": cores,
+                         "kind": "port", "sample": f"the 900-cell batch x {reps_c} ({total:.1f} s, one OpenMP "
+                                                   "worker per core)"},
+        "cpu_single_core": {"value": round(reps_1 * batch.ncells / total_1, 1), "unit": "solves/s", "cores": 1,
+                            "kind": "port", "sample": f"the 900-cell batch x {reps_1} ({total_1:.1f} s)"}})
+
+
 def bench_c3(args):
-    """Wide-range rates (1e-20..1e4), the same 10,000-cell size mix, seed 3: the mixed solver with
-    every size class on the FP32 path (no tuning), then autotuned; per-cell error against the
-    oracle, refinement-step histogram, fallback rate."""
+    """Wide-range rates (inside 1e-20..1e4) with 2-norm condition numbers 1e6..1e14 (histogram
+    logged), the same 10,000-cell size mix, seed 3: the mixed solver with every size class on the
+    FP32 path (no tuning), then autotuned; per-cell error against the oracle, refinement-step
+    histogram, fallback rate."""
     import torch
     import oracle
     import python_amd as pa
@@ -129,8 +189,9 @@ def bench_c3(args):
     A, B = pa.to_device(batch)
     oracle.build()
     Xo, co = oracle.solve_batch(batch.n, batch.offA, batch.offV, batch.A, batch.B, nthreads=_cores())
-    out = {"metric": "solves/s, wide-range rate matrices (1e-20..1e4), mixed precision", "unit": "solves/s",
-           "n_gpus": 1, "data": "synthetic (seeded ionization-balance generator, wide rates, seed 3)",
+    out = {"metric": "solves/s, wide-range rate matrices (1e-20..1e4, kappa 1e6..1e14), mixed precision",
+           "unit": "solves/s", "n_gpus": 1,
+           "data": "synthetic (seeded ionization-balance generator, wide rates, kappa targeted, seed 3)",
            "config": {"workload": f"uneven_wide_10k: {batch.ncells} cells, n in {{12..256}}"}, "modes": {}}
     for mode, tune in (("fp32_refined_untuned", False), ("autotuned", True)):
         plan = pa.Plan.for_batch(batch, mode="mixed", graph=True)
@@ -161,6 +222,9 @@ def bench_c3(args):
         plan.close()
     out["value"] = out["modes"]["autotuned"]["solves_per_s"]
     out["singular_cells_code4"] = int((co == 4).sum())
+    out["kappa_hist"] = batch.meta["kappa_hist"]
+    out["log10_kappa_range"] = [round(v, 3) for v in batch.meta["log10_kappa_range"]]
+    out["abs_entry_range"] = batch.meta["abs_entry_range"]
     return _emit(out)
 
 
@@ -199,7 +263,7 @@ def bench_c4(args):
         "config": {"workload": f"macro_inverse: {batch.ncells} cells, n in {{256,512,1024}} thirds"},
         "codes_nonzero": int((ch != 0).sum()),
         "rel_frobenius_err_sample": errs, "max_rel_frobenius_err_sample": max(max(v) for v in errs.values()),
-        "roofline": {"bound": "mfma", "achieved": round(tf, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64", "achieved": round(tf, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(tf / FP64_PEAK_TFLOPS, 5), "hbm_achieved_gbs": round(byts / (ms * 1e-3) / 1e9, 2)}})
 
 
@@ -256,8 +320,9 @@ def main():
     ap.add_argument("--quick", action="store_true", help="1/10 of the batch (development only)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="plan set-up, graph timing and the dominant-kernel measurement only (profiler runs)")
-    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5"],
-                    help="c2: the headline batch (default); c3: wide-range rates, refinement statistics; "
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+                    help="c1: 900 x n = 30; c2: the headline batch (default); c3: wide-range rates, "
+                         "refinement statistics; "
                          "c4: macro-atom inversion; c5: T_e gradient (forward solve + adjoint)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -271,7 +336,7 @@ def main():
     from python_amd.generators import uneven
 
     if args.config != "c2":
-        return {"c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.config](args)
+        return {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.config](args)
 
     t_gen = time.time()
     batch = uneven(scale=0.1 if args.quick else 1.0)

@@ -48,6 +48,14 @@ int hipla_finalize (void);
 /* Text of the last library-level failure in this process (code 5 above), or "". */
 const char *hipla_last_error_detail (void);
 
+/* Device resources the library has made in this process (no reference counterpart): the number
+ * of device allocations, streams and events created so far, and the bytes of device memory it
+ * holds now.  The drop-in entry points keep their staging buffers and a small cache of
+ * per-order plans, so once every order a caller uses has been seen, solve_matrix and
+ * invert_matrix allocate nothing and device_allocations stops growing.  Either pointer may be
+ * NULL.  Returns 0. */
+int hipla_alloc_stats (int64_t *device_allocations, int64_t *device_bytes_live);
+
 /* ------------------------------------------------------------------------------------------ *
  * Batched ragged API (new; the reference solves one matrix per call).  It batches the per-cell *
  * loops that call the path: wind_update's cell loop (source/wind_updates2d.c:89-153 ->        *
@@ -98,14 +106,15 @@ int hipla_solve (hipla_plan plan, const double *dA, const double *dB, double *dX
                  hipla_cell_stats *dstats, void *stream);
 
 /* Invert every cell (offV is ignored); dInv has A's layout.  dcodes gets invert_matrix's codes
- * per cell.  HIPLA_MODE_MIXED uses BF16x3 MFMA products for the blocked LU and inverse and
- * refines the inverse with FP64 residuals. */
+ * per cell.  Every mode runs the FP64 inverse (GSL's LU order, within 1e-8 relative Frobenius of
+ * the reference's gsl_linalg_LU_invx); dstats is not written. */
 int hipla_invert (hipla_plan plan, const double *dA, double *dInv, int *dcodes, hipla_cell_stats *dstats,
                   void *stream);
 
 /* Adjoint of the last hipla_solve on this plan (needs HIPLA_FLAG_KEEP_FACTORS, HIPLA_MODE_FP64):
  * solves A^T lambda = g per cell with the stored LU factors and writes dL/dA = -lambda x^T
- * (layout of A) into dGradA when it is not NULL.  dG, dLambda have B's layout. */
+ * (layout of A) into dGradA when it is not NULL.  dG, dLambda have B's layout.  A singular cell
+ * (solve code 4) gets lambda = 0 and dL/dA = 0. */
 int hipla_adjoint (hipla_plan plan, const double *dX, const double *dG, double *dLambda, double *dGradA,
                    void *stream);
 

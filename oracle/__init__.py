@@ -42,8 +42,6 @@ def lib():
         L = ctypes.CDLL(_LIB)
         L.ref_lu_decomp.argtypes = [_dp, ctypes.c_int, _szp, _up, _ip]
         L.ref_lu_decomp_rightlooking.argtypes = [_dp, ctypes.c_int, _szp, _up, _ip]
-        L.ref_skip_window.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip]
-        L.ref_skip_window.restype = None
         L.ref_lu_solve.argtypes = [_dp, ctypes.c_int, _szp, _dp, _dp]
         L.ref_residual_code.argtypes = [_dp, ctypes.c_int, _dp, _dp]
         L.ref_solve_matrix.argtypes = [_dp, _dp, ctypes.c_int, _dp]
@@ -75,12 +73,6 @@ def lu_decomp(a, rightlooking: bool = False):
     f = lib().ref_lu_decomp_rightlooking if rightlooking else lib().ref_lu_decomp
     f(_d(a), n, perm.ctypes.data_as(_szp), ipiv.ctypes.data_as(_up), ctypes.byref(s))
     return a, perm.astype(np.int64), ipiv.astype(np.int64), s.value
-
-
-def skip_window(n: int, k: int, pos: int):
-    lo, hi = ctypes.c_int(), ctypes.c_int()
-    lib().ref_skip_window(n, k, pos, ctypes.byref(lo), ctypes.byref(hi))
-    return lo.value, hi.value
 
 
 def lu_solve(lu, perm, b):

@@ -283,38 +283,7 @@ ref_lu_decomp (double *a, int n, size_t *perm, unsigned *ipiv, int *signum)
  * agree with the recursion.  The recursion's dgemm (cblas/source_gemm_r.h:76) additionally skips
  * exactly-zero multipliers while its dger / dtrsm do not; that decides only signs of zero results
  * and whether inf*0 produces a NaN.  This restatement (like the GPU kernels) always applies the
- * update.  ref_skip_window() gives, for step k and row position pos, the column window [lo, hi)
- * in which the recursion would reach a(pos, j) through dgemm (lo = end of k's leaf panel, hi =
- * the first split boundary on k's path above pos); it documents the rule and is exercised by the
- * tests, but no kernel uses it. */
-void
-ref_skip_window (int n, int k, int pos, int *lo, int *hi)
-{
-  int c = 0, N = n, leafend, jstar = n;
-  int bounds[32], nb = 0;
-  while (N > REF_CROSSOVER)
-  {
-    int N1 = REF_SPLIT (N);
-    int b = c + N1;
-    if (k < b)
-    {
-      bounds[nb++] = b;
-      N = N1;
-    }
-    else
-    {
-      c = b;
-      N = N - N1;
-    }
-  }
-  leafend = c + N;
-  for (int t = 0; t < nb; t++)
-    if (bounds[t] > pos && bounds[t] < jstar)
-      jstar = bounds[t];
-  *lo = leafend;
-  *hi = jstar;
-}
-
+ * update. */
 int
 ref_lu_decomp_rightlooking (double *a, int n, size_t *perm, unsigned *ipiv, int *signum)
 {

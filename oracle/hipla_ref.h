@@ -6,7 +6,6 @@
 
 int ref_lu_decomp (double *a, int n, size_t *perm, unsigned *ipiv, int *signum);
 int ref_lu_decomp_rightlooking (double *a, int n, size_t *perm, unsigned *ipiv, int *signum);
-void ref_skip_window (int n, int k, int pos, int *lo, int *hi);
 int ref_lu_solve (const double *lu, int n, const size_t *perm, const double *b, double *x);
 int ref_residual_code (const double *a_orig, int n, const double *b, const double *x);
 int ref_solve_matrix (double *a, const double *b, int n, double *x);

@@ -4,8 +4,8 @@ Drop-in for the reference's ``solve_matrix`` / ``invert_matrix`` path (source/ma
 batched ragged API, an equilibrated mixed-precision solver and an adjoint for d(x)/d(T_e).
 """
 from ._lib import HiplaError, LIB_PATH, lib  # noqa: F401
-from .solver import (Plan, get_matrix_error_string, invert_matrix, last_error_detail, solve_batch,  # noqa: F401
-                     solve_matrix, stats_to_numpy, to_device)
+from .solver import (Plan, alloc_stats, get_matrix_error_string, invert_matrix, last_error_detail,  # noqa: F401
+                     solve_batch, solve_matrix, stats_to_numpy, to_device)
 
-__all__ = ["Plan", "solve_matrix", "invert_matrix", "get_matrix_error_string", "last_error_detail", "solve_batch",
+__all__ = ["Plan", "alloc_stats", "solve_matrix", "invert_matrix", "get_matrix_error_string", "last_error_detail", "solve_batch",
            "stats_to_numpy", "to_device", "HiplaError", "lib", "LIB_PATH"]

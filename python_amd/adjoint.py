@@ -14,7 +14,7 @@ The reference obtains T_e from a root find around the solve (``calc_te``, source
 ``newton_te`` is one Newton step on T_e per cell driven by the adjoint gradient.
 
 A plan keeps the factors of its latest solve only: run backward before the next forward on the
-same plan.
+same plan (backward raises if another solve ran in between).
 """
 from __future__ import annotations
 
@@ -28,8 +28,11 @@ from .solver import Plan
 class SolveFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, A, B, plan):
-        X, codes, _ = plan.solve(A.detach().contiguous(), B.detach().contiguous())
+        # X starts at zero: a singular cell (code 4) is not written by the solve and reads 0
+        Bd = B.detach().contiguous()
+        X, codes, _ = plan.solve(A.detach().contiguous(), Bd, torch.zeros_like(Bd))
         ctx.plan = plan
+        ctx.generation = plan.generation
         ctx.save_for_backward(X)
         ctx.mark_non_differentiable(codes)
         return X, codes
@@ -38,9 +41,14 @@ class SolveFn(torch.autograd.Function):
     def backward(ctx, gX, gcodes):
         (X,) = ctx.saved_tensors
         plan = ctx.plan
+        if plan.generation != ctx.generation:
+            raise RuntimeError("the plan was solved again after this forward pass: its kept LU factors belong to a "
+                               "later solve, so this backward would be wrong (run backward first, or use one plan "
+                               "per forward)")
         Lam = torch.empty_like(X)
         need_A = ctx.needs_input_grad[0]
         GradA = torch.empty(plan.totA, dtype=X.dtype, device=X.device) if need_A else None
+        # singular cells (code 4) get lambda = 0 and dL/dA = 0 from the adjoint kernel
         plan.adjoint(X, gX.contiguous(), Lam, GradA)
         return GradA, Lam, None
 

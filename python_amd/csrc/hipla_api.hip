@@ -9,12 +9,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/hipla.h"
@@ -95,16 +97,80 @@ extern "C" int hipla_init (void)
 }
 
 // --------------------------------------------------------------------------------------------
-// drop-in single-matrix context: one pooled plan, grown as needed
+// allocation accounting
 // --------------------------------------------------------------------------------------------
+static std::atomic<long long> g_allocs{ 0 }, g_live_bytes{ 0 };
+static std::mutex g_size_mu;
+static std::unordered_map<void *, size_t> g_sizes;   // live device allocations and their sizes
+
+hipError_t hipla_dev_alloc (void **ptr, size_t bytes)
+{
+  hipError_t e = hipMalloc (ptr, bytes);
+  if (e == hipSuccess)
+  {
+    g_allocs++;
+    g_live_bytes += (long long) bytes;
+    std::lock_guard<std::mutex> lk (g_size_mu);
+    g_sizes[*ptr] = bytes;
+  }
+  return e;
+}
+
+hipError_t hipla_dev_free (void *ptr)
+{
+  if (!ptr)
+    return hipSuccess;
+  {
+    std::lock_guard<std::mutex> lk (g_size_mu);
+    auto it = g_sizes.find (ptr);
+    if (it != g_sizes.end ())
+    {
+      g_live_bytes -= (long long) it->second;
+      g_sizes.erase (it);
+    }
+  }
+  return hipFree (ptr);
+}
+
+void hipla_count_resource ()
+{
+  g_allocs++;
+}
+
+extern "C" int hipla_alloc_stats (int64_t *device_allocations, int64_t *device_bytes_live)
+{
+  if (device_allocations)
+    *device_allocations = (int64_t) g_allocs.load ();
+  if (device_bytes_live)
+    *device_bytes_live = (int64_t) g_live_bytes.load ();
+  return 0;
+}
+
+// --------------------------------------------------------------------------------------------
+// drop-in single-matrix context: staging buffers grown to the largest order seen, and a small
+// least-recently-used cache of one-cell FP64 plans keyed by the order alone (solve and invert
+// share a plan; its inverse workspace is made on the first invert).  The reference's callers
+// cycle through a few orders -- macro_pops solves one system per macro-atom element
+// (source/macro_gov.c:366-444) between the ionization solves (source/matrix_ion.c:276) and the
+// macro-atom inversions (source/macro_accelerate.c:315) -- so after the first pass over them a
+// call allocates nothing.
+// --------------------------------------------------------------------------------------------
+struct DropPlan
+{
+  int n = -1;
+  hipla_plan plan = nullptr;
+  unsigned long long used = 0;
+};
+
 struct DropIn
 {
+  static constexpr int kPlans = 8;
   hipStream_t stream = nullptr;
   double *dA = nullptr, *dB = nullptr, *dX = nullptr;
   int *dcode = nullptr;
   size_t capA = 0, capV = 0;
-  hipla_plan plan = nullptr;
-  int plan_n = -1, plan_kind = -1;
+  DropPlan plans[kPlans];
+  unsigned long long tick = 0;
 };
 static DropIn g_drop;
 
@@ -112,42 +178,64 @@ static int drop_reserve (int n)
 {
   const size_t nn = (size_t) n * n, nv = (size_t) n;
   if (!g_drop.stream)
+  {
     HIPLA_CHECK (hipStreamCreateWithFlags (&g_drop.stream, hipStreamNonBlocking));
+    hipla_count_resource ();
+  }
   if (!g_drop.dcode)
-    HIPLA_CHECK (hipMalloc (&g_drop.dcode, sizeof (int) * 2));
+    HIPLA_CHECK (hipla_dev_alloc (&g_drop.dcode, sizeof (int) * 2));
   if (nn > g_drop.capA)
   {
     if (g_drop.dA)
-      HIPLA_CHECK (hipFree (g_drop.dA));
+      HIPLA_CHECK (hipla_dev_free (g_drop.dA));
     g_drop.dA = nullptr;
-    HIPLA_CHECK (hipMalloc (&g_drop.dA, sizeof (double) * nn * 2));
+    HIPLA_CHECK (hipla_dev_alloc (&g_drop.dA, sizeof (double) * nn * 2));
     g_drop.capA = nn;
   }
   if (nv > g_drop.capV)
   {
     if (g_drop.dB)
-      HIPLA_CHECK (hipFree (g_drop.dB));
+      HIPLA_CHECK (hipla_dev_free (g_drop.dB));
     g_drop.dB = nullptr;
-    HIPLA_CHECK (hipMalloc (&g_drop.dB, sizeof (double) * nv * 2));
+    HIPLA_CHECK (hipla_dev_alloc (&g_drop.dB, sizeof (double) * nv * 2));
     g_drop.dX = g_drop.dB + nv;
     g_drop.capV = nv;
   }
   return 0;
 }
 
-static int drop_plan (int n, int kind)
+// The cached plan for order n (made on a miss, replacing the least recently used one).
+static int drop_plan (int n, hipla_plan &out)
 {
-  if (g_drop.plan && g_drop.plan_n == n && g_drop.plan_kind == kind)
-    return 0;
-  if (g_drop.plan)
-    hipla_plan_destroy (g_drop.plan);
-  g_drop.plan = nullptr;
-  int64_t off = 0;
-  int rc = hipla_plan_create (&g_drop.plan, 1, &n, &off, &off, HIPLA_MODE_FP64, 0);
-  if (rc)
-    return rc;
-  g_drop.plan_n = n;
-  g_drop.plan_kind = kind;
+  DropPlan *slot = nullptr;
+  for (DropPlan &d : g_drop.plans)
+    if (d.plan && d.n == n)
+      slot = &d;
+  if (!slot)
+  {
+    slot = &g_drop.plans[0];
+    for (DropPlan &d : g_drop.plans)
+    {
+      if (!d.plan)
+      {
+        slot = &d;
+        break;
+      }
+      if (d.used < slot->used)
+        slot = &d;
+    }
+    if (slot->plan)
+      hipla_plan_destroy (slot->plan);
+    slot->plan = nullptr;
+    slot->n = -1;
+    int64_t off = 0;
+    int rc = hipla_plan_create (&slot->plan, 1, &n, &off, &off, HIPLA_MODE_FP64, 0);
+    if (rc)
+      return rc;
+    slot->n = n;
+  }
+  slot->used = ++g_drop.tick;
+  out = slot->plan;
   return 0;
 }
 
@@ -163,12 +251,13 @@ extern "C" int solve_matrix (double *a_matrix, double *b_matrix, int size, doubl
   int rc = init_locked ();
   if (rc)
     return rc;
-  if ((rc = drop_reserve (size)) || (rc = drop_plan (size, 0)))
+  hipla_plan plan = nullptr;
+  if ((rc = drop_reserve (size)) || (rc = drop_plan (size, plan)))
     return rc;
   const size_t nn = (size_t) size * size;
   HIPLA_CHECK (hipMemcpyAsync (g_drop.dA, a_matrix, sizeof (double) * nn, hipMemcpyHostToDevice, g_drop.stream));
   HIPLA_CHECK (hipMemcpyAsync (g_drop.dB, b_matrix, sizeof (double) * size, hipMemcpyHostToDevice, g_drop.stream));
-  rc = hipla_solve (g_drop.plan, g_drop.dA, g_drop.dB, g_drop.dX, g_drop.dcode, nullptr, g_drop.stream);
+  rc = hipla_solve (plan, g_drop.dA, g_drop.dB, g_drop.dX, g_drop.dcode, nullptr, g_drop.stream);
   if (rc)
     return rc;
   int code = 5;
@@ -193,12 +282,13 @@ extern "C" int invert_matrix (double *matrix, double *inverted_matrix, int num_r
   int rc = init_locked ();
   if (rc)
     return rc;
-  if ((rc = drop_reserve (num_rows)) || (rc = drop_plan (num_rows, 1)))
+  hipla_plan plan = nullptr;
+  if ((rc = drop_reserve (num_rows)) || (rc = drop_plan (num_rows, plan)))
     return rc;
   const size_t nn = (size_t) num_rows * num_rows;
   double *dInv = g_drop.dA + nn;
   HIPLA_CHECK (hipMemcpyAsync (g_drop.dA, matrix, sizeof (double) * nn, hipMemcpyHostToDevice, g_drop.stream));
-  rc = hipla_invert (g_drop.plan, g_drop.dA, dInv, g_drop.dcode, nullptr, g_drop.stream);
+  rc = hipla_invert (plan, g_drop.dA, dInv, g_drop.dcode, nullptr, g_drop.stream);
   if (rc)
     return rc;
   int code = 5;
@@ -233,14 +323,12 @@ extern "C" const char *get_matrix_error_string (int error_code)
 extern "C" int hipla_finalize (void)
 {
   std::lock_guard<std::recursive_mutex> lk (g_mu);
-  if (g_drop.plan)
-    hipla_plan_destroy (g_drop.plan);
-  if (g_drop.dA)
-    (void) hipFree (g_drop.dA);
-  if (g_drop.dB)
-    (void) hipFree (g_drop.dB);
-  if (g_drop.dcode)
-    (void) hipFree (g_drop.dcode);
+  for (DropPlan &d : g_drop.plans)
+    if (d.plan)
+      hipla_plan_destroy (d.plan);
+  (void) hipla_dev_free (g_drop.dA);
+  (void) hipla_dev_free (g_drop.dB);
+  (void) hipla_dev_free (g_drop.dcode);
   if (g_drop.stream)
     (void) hipStreamDestroy (g_drop.stream);
   g_drop = DropIn ();

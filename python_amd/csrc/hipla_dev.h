@@ -1,7 +1,7 @@
 // hipla_dev.h -- device helpers shared by the gfx950 kernels of libhipla.
 //
-// Wave64 reductions, the pivot key of GSL's idamax, the zero-multiplier skip window of GSL's
-// recursive LU, and the argument blocks the host side passes to the kernels.
+// Wave64 reductions, the pivot key of GSL's idamax, GSL's multiplier rule, and the argument
+// blocks the host side passes to the kernels.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -162,45 +162,6 @@ __device__ __forceinline__ void wave_argmax_keypos32 (int &key, int &pos)
 __device__ __forceinline__ void wave_lds_fence ()
 {
   __asm__ volatile ("s_waitcnt lgkmcnt(0)" ::: "memory");
-}
-
-// ----------------------------------------------------------------------------------------
-// Zero-multiplier skip window of GSL's recursive LU (linalg/lu.c:204-288, recurse.h:2,6-7).
-//
-// GSL reaches a(pos, j) from step k through dger inside a leaf panel, dtrsm above a split
-// boundary, or dgemm below it; only dgemm (cblas/source_gemm_r.h:76) skips a zero multiplier.
-// For step k the dgemm columns of row position pos are [lo, hi): lo = end of k's leaf panel,
-// hi = the first split boundary on k's path above pos (n if none).  Skipping only preserves
-// signs of zeros and keeps inf*0 out, so the rule matters for bit patterns, not values.
-// ----------------------------------------------------------------------------------------
-__device__ __forceinline__ int gsl_split (int n)
-{
-  return (n >= 16) ? ((n + 8) / 16) * 8 : n / 2;
-}
-
-// Walk k's path down the recursion.  Boundaries where k goes left are met in decreasing order,
-// so the last one above pos is the smallest one above pos.
-__device__ __forceinline__ void skip_window (int n, int k, int pos, int &lo, int &hi)
-{
-  int c = 0, N = n, h = n;
-  while (N > 24)
-  {
-    const int N1 = gsl_split (N);
-    const int b = c + N1;
-    if (k < b)
-    {
-      if (b > pos)
-        h = b;
-      N = N1;
-    }
-    else
-    {
-      c = b;
-      N -= N1;
-    }
-  }
-  lo = c + N;
-  hi = h;
 }
 
 // GSL's multiplier: reciprocal multiply when |pivot| >= DBL_MIN, else divide (linalg/lu.c:162-176).

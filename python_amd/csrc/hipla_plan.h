@@ -20,6 +20,18 @@ const DeviceInfo &hipla_device_info ();
 int hipla_ensure_init ();
 void hipla_set_detail (const char *fmt, ...);
 
+// Device allocations of the library go through these two (hipMalloc / hipFree, counted), and
+// every stream or event it creates is counted by hipla_count_resource: hipla_alloc_stats reports
+// the totals, so a caller can see that warm calls allocate nothing.
+hipError_t hipla_dev_alloc (void **ptr, size_t bytes);
+hipError_t hipla_dev_free (void *ptr);
+void hipla_count_resource ();
+template <typename T>
+inline hipError_t hipla_dev_alloc (T **ptr, size_t bytes)
+{
+  return hipla_dev_alloc (reinterpret_cast<void **> (ptr), bytes);
+}
+
 #define HIPLA_CHECK(expr)                                                                        \
   do                                                                                             \
   {                                                                                              \

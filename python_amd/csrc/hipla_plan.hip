@@ -8,6 +8,8 @@
 // sequence is captured once in a hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
+#include <stdint.h>
+
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -81,7 +83,7 @@ static int free_plan_device (hipla_plan p)
                    p->d_LU,  p->d_piv,  p->own_F ? p->d_F : nullptr, p->d_Y, p->d_lam };
   for (void *b : bufs)
     if (b)
-      (void) hipFree (b);
+      (void) hipla_dev_free (b);
   for (auto s : p->aux)
     (void) hipStreamDestroy (s);
   p->aux.clear ();
@@ -118,6 +120,19 @@ extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, con
     hipla_set_detail ("hipla_plan_create: unknown mode %d", mode);
     return 5;
   }
+  // every cell's extent is checked before anything else (so offA + n^2 cannot wrap: the offsets
+  // are compared with the room the int64 extents leave)
+  for (int c = 0; c < ncells; c++)
+  {
+    const int64_t k = n[c];
+    if (k <= 0 || k > 65536 || offA[c] < 0 || offV[c] < 0 || offA[c] > INT64_MAX - k * k ||
+        offV[c] > INT64_MAX - k)
+    {
+      hipla_set_detail ("hipla_plan_create: cell %d has n=%d offA=%lld offV=%lld", c, n[c], (long long) offA[c],
+                        (long long) offV[c]);
+      return 5;
+    }
+  }
   int rc = hipla_ensure_init ();
   if (rc)
     return rc;
@@ -131,13 +146,6 @@ extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, con
   p->offV.assign (offV, offV + ncells);
   for (int c = 0; c < ncells; c++)
   {
-    if (n[c] <= 0 || n[c] > 65536 || offA[c] < 0 || offV[c] < 0)
-    {
-      hipla_set_detail ("hipla_plan_create: cell %d has n=%d offA=%lld offV=%lld", c, n[c], (long long) offA[c],
-                        (long long) offV[c]);
-      delete p;
-      return 5;
-    }
     p->totA = std::max<int64_t> (p->totA, offA[c] + (int64_t) n[c] * n[c]);
     p->totV = std::max<int64_t> (p->totV, offV[c] + n[c]);
   }
@@ -210,10 +218,10 @@ extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, con
       return 5;                                                                            \
     }                                                                                      \
   } while (0)
-  PCHECK (hipMalloc (&p->d_n, sizeof (int) * ncells));
-  PCHECK (hipMalloc (&p->d_offA, sizeof (int64_t) * ncells));
-  PCHECK (hipMalloc (&p->d_offV, sizeof (int64_t) * ncells));
-  PCHECK (hipMalloc (&p->d_list, sizeof (int) * ncells));
+  PCHECK (hipla_dev_alloc (&p->d_n, sizeof (int) * ncells));
+  PCHECK (hipla_dev_alloc (&p->d_offA, sizeof (int64_t) * ncells));
+  PCHECK (hipla_dev_alloc (&p->d_offV, sizeof (int64_t) * ncells));
+  PCHECK (hipla_dev_alloc (&p->d_list, sizeof (int) * ncells));
   PCHECK (hipMemcpy (p->d_n, n, sizeof (int) * ncells, hipMemcpyHostToDevice));
   PCHECK (hipMemcpy (p->d_offA, offA, sizeof (int64_t) * ncells, hipMemcpyHostToDevice));
   PCHECK (hipMemcpy (p->d_offV, offV, sizeof (int64_t) * ncells, hipMemcpyHostToDevice));
@@ -222,16 +230,16 @@ extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, con
     std::vector<int> desc (ncells);
     std::iota (desc.begin (), desc.end (), 0);
     std::stable_sort (desc.begin (), desc.end (), [&] (int a, int b) { return n[a] > n[b]; });
-    PCHECK (hipMalloc (&p->d_list_desc, sizeof (int) * ncells));
+    PCHECK (hipla_dev_alloc (&p->d_list_desc, sizeof (int) * ncells));
     PCHECK (hipMemcpy (p->d_list_desc, desc.data (), sizeof (int) * ncells, hipMemcpyHostToDevice));
   }
   if (mode == HIPLA_MODE_MIXED)
-    PCHECK (hipMalloc (&p->d_fallback, sizeof (int) * 5 * (ncells + 1)));
+    PCHECK (hipla_dev_alloc (&p->d_fallback, sizeof (int) * 5 * (ncells + 1)));
   if (flags & HIPLA_FLAG_KEEP_FACTORS)
   {
-    PCHECK (hipMalloc (&p->d_LU, sizeof (double) * p->totA));
-    PCHECK (hipMalloc (&p->d_piv, sizeof (int) * p->totV));
-    PCHECK (hipMalloc (&p->d_lam, sizeof (double) * p->totV));
+    PCHECK (hipla_dev_alloc (&p->d_LU, sizeof (double) * p->totA));
+    PCHECK (hipla_dev_alloc (&p->d_piv, sizeof (int) * p->totV));
+    PCHECK (hipla_dev_alloc (&p->d_lam, sizeof (double) * p->totV));
   }
   // factor and y stores of the workgroup factor kernels (cells above 64), read by the
   // back-substitution kernel; kept factors double as the factor store
@@ -245,10 +253,10 @@ extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, con
         p->d_F = p->d_LU;
       else
       {
-        PCHECK (hipMalloc (&p->d_F, sizeof (double) * p->totA));
+        PCHECK (hipla_dev_alloc (&p->d_F, sizeof (double) * p->totA));
         p->own_F = true;
       }
-      PCHECK (hipMalloc (&p->d_Y, sizeof (double) * p->totV));
+      PCHECK (hipla_dev_alloc (&p->d_Y, sizeof (double) * p->totV));
     }
   }
   p->aux.resize (1, nullptr);
@@ -256,6 +264,8 @@ extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, con
   PCHECK (hipStreamCreateWithFlags (&p->fork_st, hipStreamNonBlocking));
   PCHECK (hipEventCreateWithFlags (&p->ev_fork, hipEventDisableTiming));
   PCHECK (hipEventCreateWithFlags (&p->ev_join, hipEventDisableTiming));
+  for (int i = 0; i < 4; i++)
+    hipla_count_resource ();
 #undef PCHECK
   *out = p;
   return 0;
@@ -267,6 +277,18 @@ extern "C" int hipla_plan_set_refinement (hipla_plan plan, int max_refine, doubl
   {
     hipla_set_detail ("hipla_plan_set_refinement: bad argument");
     return 5;
+  }
+  // the cached graph holds kernel arguments captured with the old settings: drop it so the next
+  // solve captures the new ones
+  if (plan->max_refine != max_refine || plan->tol != tol)
+  {
+    if (plan->gexec)
+      (void) hipGraphExecDestroy (plan->gexec);
+    if (plan->graph)
+      (void) hipGraphDestroy (plan->graph);
+    plan->gexec = nullptr;
+    plan->graph = nullptr;
+    plan->gkind = -1;
   }
   plan->max_refine = max_refine;
   plan->tol = tol;

@@ -71,6 +71,8 @@ struct EvPair
   {
     (void) hipEventCreate (&a);
     (void) hipEventCreate (&b);
+    hipla_count_resource ();
+    hipla_count_resource ();
   }
   ~EvPair ()
   {

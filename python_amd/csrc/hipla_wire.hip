@@ -62,8 +62,11 @@ bool extents_ok (int64_t ncells, const int *n, const int64_t *offA, const int64_
 {
   for (int64_t c = 0; c < ncells; c++)
   {
+    // no sum that can wrap: the offsets are compared with the room left below the totals, and n
+    // is held to what hipla_plan_create accepts
     const int64_t k = n[c];
-    if (k < 1 || offA[c] < 0 || offV[c] < 0 || offA[c] + k * k > totA || offV[c] + k > totV)
+    if (k < 1 || k > 65536 || offA[c] < 0 || offV[c] < 0 || k * k > totA || offA[c] > totA - k * k || k > totV ||
+        offV[c] > totV - k)
     {
       hipla_set_detail ("%s: cell %lld (n = %lld) lies outside the batch", who, (long long) c, (long long) k);
       return false;
@@ -89,9 +92,9 @@ extern "C" int hipla_batch_write (const char *path, int ncells, const int *n, co
   for (int c = 0; c < ncells; c++)
   {
     const int64_t k = n[c];
-    if (k < 1 || offA[c] < 0 || offV[c] < 0)
+    if (k < 1 || k > 65536 || offA[c] < 0 || offV[c] < 0 || offA[c] > INT64_MAX - k * k || offV[c] > INT64_MAX - k)
     {
-      hipla_set_detail ("hipla_batch_write: cell %d has n = %d or a negative offset", c, n[c]);
+      hipla_set_detail ("hipla_batch_write: cell %d has n = %d or an offset out of range", c, n[c]);
       return 5;
     }
     h.totA = std::max (h.totA, offA[c] + k * k);

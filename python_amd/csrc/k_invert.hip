@@ -316,11 +316,11 @@ int plan_invert_prepare (hipla_plan p)
   if (!p->d_inv_work || p->inv_wstride < need || p->inv_slots < grid)
   {
     if (p->d_inv_work)
-      HIPLA_CHECK (hipFree (p->d_inv_work));
+      HIPLA_CHECK (hipla_dev_free (p->d_inv_work));
     p->d_inv_work = nullptr;
     p->inv_wstride = (need + 63) & ~(int64_t) 63;
     p->inv_slots = grid;
-    HIPLA_CHECK (hipMalloc (&p->d_inv_work, sizeof (double) * p->inv_wstride * grid));
+    HIPLA_CHECK (hipla_dev_alloc (&p->d_inv_work, sizeof (double) * p->inv_wstride * grid));
   }
   return 0;
 }
@@ -377,6 +377,20 @@ __global__ __launch_bounds__ (64) void k_adjoint_f64 (const int *nv, const int64
     const double *F = LU + offA[cell];
     const int *pv = ipiv + offV[cell];
     const double *g = G + offV[cell];
+    // a singular cell (an exact zero on U's diagonal, solve code 4) has no adjoint: lambda and
+    // dL/dA are written as zeros instead of the infinities a division by the zero pivot gives
+    bool zpiv = false;
+    for (int i = lane; i < n; i += 64)
+      zpiv |= F[(size_t) i * n + i] == 0.0;
+    if (__ballot (zpiv))
+    {
+      for (int i = lane; i < n; i += 64)
+        Lam[offV[cell] + i] = 0.0;
+      if (GradA)
+        for (int64_t e = lane; e < (int64_t) n * n; e += 64)
+          GradA[offA[cell] + e] = 0.0;
+      continue;
+    }
     double v[MAXM];
 #pragma unroll
     for (int m = 0; m < MAXM; m++)

@@ -9,7 +9,8 @@ generators restate their *structure* with closed-form random rates:
   i+1 -> i); every diagonal is minus its column sum (population conservation of the rate
   equations); then the row of each element's first ion is replaced by the abundance constraint
   (ones over that element's ions, b = abundance), as ``populate_ion_rate_matrix`` does
-  (matrix_ion.c:633-655).  b is zero except on those rows.
+  (matrix_ion.c:633-655).  b is zero except on those rows.  With ``auger=True`` (the uneven
+  batches) inner-shell ionization adds A[i+d, i] for jumps d = 1..10 (matrix_ion.c:600-616).
 * Macro-atom inverse (``macro_accelerate.c:269-284``): N = I - Q with Q substochastic.
 * Macro-atom level populations (``macro_gov.c:366-440``, ``macro_pops_fill_rate_matrix``
   ``:535-749``): per element a rate matrix over its levels with M[to, from] += rate and
@@ -62,9 +63,41 @@ def _logu(rng, lo, hi, size):
     return np.exp(rng.uniform(np.log(lo), np.log(hi), size))
 
 
+AUGER_MAX_JUMP = 10     # inner-shell ionization ejects at most 10 extra electrons (n_elec <= 11)
+AUGER_SCALE = 1e-2      # inner-shell rates relative to the ion's outer-shell rate range
+
+
+def auger_sites(blocks):
+    """(column, first row, jumps) of every ion with an inner shell that can autoionize: an ion with
+    n_elec >= 3 bound electrons, whose inner-shell ionization lands d = 1 .. min(n_elec, 11) - 1
+    stages higher (source/matrix_ion.c:600-616: rate_matrix[ion + d][ion] += rate * yield[d-1],
+    n_elec = z - istate + 1 capped at 11).  Element blocks of m stages hold z = m - 1, so stage
+    k (istate k + 1) has n_elec = m - 1 - k."""
+    out = []
+    for (s, m, ab) in blocks:
+        for k in range(m):
+            n_elec = min(m - 1 - k, AUGER_MAX_JUMP + 1)
+            if n_elec >= 3:
+                out.append((s + k, s + k + 1, n_elec - 1))
+    return out
+
+
+def _add_auger(rng, A, blocks, rates):
+    """Add the inner-shell terms: rates[:, t] for site t, spread over its jumps by yields that sum
+    to one (a seeded Dirichlet draw), into column `col` below the diagonal."""
+    for t, (col, r0, nj) in enumerate(auger_sites(blocks)):
+        y = rng.gamma(1.0, 1.0, (A.shape[0], nj))
+        y /= y.sum(axis=1, keepdims=True)
+        A[:, r0:r0 + nj, col] += rates[:, t:t + 1] * y
+
+
 def ion_matrices(rng, ncells: int, n: int, ion_range=(1e-12, 1e2), ne_range=(1e6, 1e14),
-                 alpha_range=(1e-14, 1e-10)):
-    """ncells ionization-balance matrices of order n: returns (A [ncells,n,n], b [ncells,n])."""
+                 alpha_range=(1e-14, 1e-10), auger: bool = False):
+    """ncells ionization-balance matrices of order n: returns (A [ncells,n,n], b [ncells,n]).
+
+    auger=True adds inner-shell (Auger) ionization: every ion with three or more electrons gets
+    a rate log-uniform over AUGER_SCALE * ion_range, split over jumps of +1 .. +10 stages, so
+    the element blocks fill below the sub-diagonal as the reference's do."""
     A = np.zeros((ncells, n, n))
     b = np.zeros((ncells, n))
     ne = _logu(rng, *ne_range, ncells)
@@ -76,6 +109,10 @@ def ion_matrices(rng, ncells: int, n: int, ion_range=(1e-12, 1e2), ne_range=(1e6
             rec = ne[:, None] * _logu(rng, *alpha_range, (ncells, m - 1))
             A[:, s + k + 1, s + k] = ion
             A[:, s + k, s + k + 1] = rec
+    if auger:
+        sites = auger_sites(blocks)
+        lo, hi = AUGER_SCALE * ion_range[0], AUGER_SCALE * ion_range[1]
+        _add_auger(rng, A, blocks, _logu(rng, lo, hi, (ncells, len(sites))))
     d = np.arange(n)
     A[:, d, d] = -A.sum(axis=1)
     for (s, m, ab) in blocks:
@@ -83,6 +120,92 @@ def ion_matrices(rng, ncells: int, n: int, ion_range=(1e-12, 1e2), ne_range=(1e6
         A[:, s, s:s + m] = 1.0
         b[:, s] = ab
     return A, b
+
+
+WIDE_LOG_RATES = (-20.0, 4.0)      # log10 of the wide rate range, 1e-20 .. 1e4 s^-1
+WIDE_LOG_KAPPA = (6.0, 14.0)       # log10 of the condition-number range of the wide batch
+
+
+def _contracted_ion_matrix(n, u_ion, u_rec, u_aug, yields, s, m):
+    """Ionization matrices whose log10 rates are m + s * (u + 8), clipped to the wide range: u are
+    raw log10 draws over WIDE_LOG_RATES (centre -8), s in [0, 1] contracts them toward the
+    cell's level m.  Inner-shell rates (u_aug, split over their jumps by ``yields``) sit two
+    decades lower.  One row per element is the abundance constraint (ones)."""
+    c = u_ion.shape[0]
+    A = np.zeros((c, n, n))
+    lo, hi = WIDE_LOG_RATES
+    j = 0
+    blocks = element_layout(n)
+    for (st, mm, ab) in blocks:
+        if mm > 1:
+            k = np.arange(mm - 1)
+            li = np.clip(m[:, None] + s[:, None] * (u_ion[:, j:j + mm - 1] + 8.0), lo, hi)
+            lr = np.clip(m[:, None] + s[:, None] * (u_rec[:, j:j + mm - 1] + 8.0), lo, hi)
+            A[:, st + k + 1, st + k] = 10.0 ** li
+            A[:, st + k, st + k + 1] = 10.0 ** lr
+            j += mm - 1
+    la = np.clip(m[:, None] + s[:, None] * (u_aug + 8.0) + np.log10(AUGER_SCALE), lo, hi)
+    for t, (col, r0, nj) in enumerate(auger_sites(blocks)):
+        A[:, r0:r0 + nj, col] += (10.0 ** la[:, t:t + 1]) * yields[t]
+    d = np.arange(n)
+    A[:, d, d] = -A.sum(axis=1)
+    for (st, mm, ab) in blocks:
+        A[:, st, :] = 0.0
+        A[:, st, st:st + mm] = 1.0
+    return A
+
+
+def ion_matrices_conditioned(rng, ncells: int, n: int, log_kappa=WIDE_LOG_KAPPA, iters: int = 10):
+    """ncells ionization matrices of order n with rates inside 1e-20..1e4 and a 2-norm condition
+    number log-uniform over 10^log_kappa (the wide-range precision workload).
+
+    Raw rates (ionization i -> i+1, recombination n_e alpha, i+1 -> i, and inner-shell ionization
+    with Auger jumps of +1 .. +10 stages) are drawn log-uniform over the whole 1e-20..1e4 range; those raw matrices have kappa of 1e17..1e25 and many are
+    numerically singular in FP64.  Each cell therefore draws a target kappa and a rate level m
+    (log10 uniform over -8..0), and its log-rates are contracted toward m by a factor s found by
+    bisection on numpy's 2-norm condition number.  Cells that miss the range are drawn again.
+    Returns (A, b, log10 kappa per cell)."""
+    blocks = element_layout(n)
+    nr = sum(mm - 1 for (_, mm, _) in blocks if mm > 1)
+    sites = auger_sites(blocks)
+    A = np.zeros((ncells, n, n))
+    lk = np.full(ncells, np.nan)
+    todo = np.arange(ncells)
+    for _round in range(20):
+        if len(todo) == 0:
+            break
+        c = len(todo)
+        u_ion = rng.uniform(*WIDE_LOG_RATES, (c, nr))
+        u_rec = rng.uniform(*WIDE_LOG_RATES, (c, nr))
+        u_aug = rng.uniform(*WIDE_LOG_RATES, (c, len(sites)))
+        yields = []
+        for (_col, _r0, nj) in sites:
+            y = rng.gamma(1.0, 1.0, (c, nj))
+            yields.append(y / y.sum(axis=1, keepdims=True))
+        m = rng.uniform(-8.0, 0.0, c)
+
+        def build(sv):
+            return _contracted_ion_matrix(n, u_ion, u_rec, u_aug, yields, sv, m)
+
+        k0 = np.log10(np.linalg.cond(build(np.zeros(c))))
+        t = np.maximum(rng.uniform(*log_kappa, c), k0 + 0.25)
+        slo, shi = np.zeros(c), np.ones(c)
+        for _ in range(iters):
+            s = 0.5 * (slo + shi)
+            up = np.log10(np.linalg.cond(build(s))) < t
+            slo, shi = np.where(up, s, slo), np.where(up, shi, s)
+        Ac = build(0.5 * (slo + shi))
+        kc = np.log10(np.linalg.cond(Ac))
+        ok = (kc >= log_kappa[0]) & (kc <= log_kappa[1])
+        A[todo[ok]] = Ac[ok]
+        lk[todo[ok]] = kc[ok]
+        todo = todo[~ok]
+    if len(todo):
+        raise RuntimeError(f"{len(todo)} cells of order {n} missed the condition-number range")
+    b = np.zeros((ncells, n))
+    for (st, mm, ab) in blocks:
+        b[:, st] = ab
+    return A, b, lk
 
 
 def ion_matrices_te(rng, ncells: int, n: int, te=None):
@@ -282,23 +405,38 @@ def plain_small(ncells: int = 900, seed: int = 1) -> RaggedBatch:
 
 
 def uneven(ncells: int = 10000, seed: int = 2, wide: bool = False, scale: float = 1.0) -> RaggedBatch:
-    """10,000 cells, n in {12..256} with counts 3000/2500/1500/1000/1000/500/500, shuffled.
+    """10,000 cells, n in {12..256} with counts 3000/2500/1500/1000/1000/500/500, shuffled, with
+    inner-shell (Auger) fill below each element block's sub-diagonal.
 
-    wide=True widens the rates to 1e-20..1e4 (the mixed-precision stress case, seed 3).
+    wide=True is the precision workload (seed 3): rates inside 1e-20..1e4 and condition numbers
+    log-uniform over 1e6..1e14 (``ion_matrices_conditioned``); the per-cell log10 kappa is kept in
+    ``meta["log10_kappa"]`` (cell order) with its histogram in ``meta["kappa_hist"]``.
     ``scale`` shrinks every count proportionally (for quick tests)."""
     rng = np.random.default_rng(seed)
     counts = [max(1, int(round(c * scale * ncells / 10000))) for c in UNEVEN_COUNTS]
     order = np.concatenate([np.full(c, n, dtype=np.int32) for n, c in zip(UNEVEN_SIZES, counts)])
     rng.shuffle(order)
-    groups = {}
+    groups, kappas = {}, {}
     for n, c in zip(UNEVEN_SIZES, counts):
         if wide:
-            groups[n] = ion_matrices(rng, c, n, ion_range=(1e-20, 1e4), ne_range=(1e6, 1e14),
-                                     alpha_range=(1e-26, 1e-10))
+            A, b, lk = ion_matrices_conditioned(rng, c, n)
+            groups[n] = (A, b)
+            kappas[n] = lk
         else:
-            groups[n] = ion_matrices(rng, c, n)
+            groups[n] = ion_matrices(rng, c, n, auger=True)
     batch = pack_groups(groups, order)
     batch.meta = {"workload": "uneven_wide" if wide else "uneven", "seed": seed, "counts": dict(zip(UNEVEN_SIZES, counts))}
+    if wide:
+        lk = np.empty(len(order))
+        for n in kappas:
+            lk[order == n] = kappas[n]          # pack_groups takes each size's cells in order
+        nz = batch.A[batch.A != 0.0]
+        rates = np.abs(nz[np.abs(nz) != 1.0])
+        batch.meta.update({
+            "log10_kappa": lk,
+            "kappa_hist": {f"1e{e}..1e{e + 1}": int(((lk >= e) & (lk < e + 1)).sum()) for e in range(6, 14)},
+            "log10_kappa_range": [float(lk.min()), float(lk.max())],
+            "abs_entry_range": [float(rates.min()), float(rates.max())]})
     return batch
 
 

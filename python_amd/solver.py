@@ -63,6 +63,13 @@ def last_error_detail() -> str:
     return d.decode() if d else ""
 
 
+def alloc_stats() -> tuple[int, int]:
+    """(device allocations + streams + events made so far, device bytes held now) of libhipla."""
+    n, live = ctypes.c_int64(), ctypes.c_int64()
+    check(lib().hipla_alloc_stats(ctypes.byref(n), ctypes.byref(live)), "hipla_alloc_stats")
+    return n.value, live.value
+
+
 def _ptr(t):
     return ctypes.c_void_p(0 if t is None else int(t.data_ptr()))
 
@@ -102,6 +109,7 @@ class Plan:
                                       self.offA.ctypes.data_as(_lib._i64p), self.offV.ctypes.data_as(_lib._i64p),
                                       _MODES[mode], flags), "hipla_plan_create")
         self._h = h
+        self.generation = 0    # solves issued on this plan (the adjoint uses the latest one's factors)
         self.totA = int((self.offA + self.n.astype(np.int64) ** 2).max()) if self.ncells else 0
         self.totV = int((self.offV + self.n).max()) if self.ncells else 0
 
@@ -120,7 +128,7 @@ class Plan:
         except Exception:
             pass
 
-    def set_refinement(self, max_refine: int = 5, tol: float = 1e-13):
+    def set_refinement(self, max_refine: int = 5, tol: float = 1e-12):
         check(lib().hipla_plan_set_refinement(self._h, int(max_refine), float(tol)), "hipla_plan_set_refinement")
 
     def describe(self) -> dict:
@@ -153,6 +161,7 @@ class Plan:
             stats = None
         check(lib().hipla_solve(self._h, _ptr(A), _ptr(B), _ptr(X), _ptr(codes), _ptr(stats), _stream_handle(stream)),
               "hipla_solve")
+        self.generation += 1
         return X, codes, stats
 
     def invert(self, A, Inv=None, codes=None, stats=None, stream=None):

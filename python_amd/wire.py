@@ -31,13 +31,31 @@ VERSION = 1
 _HEADER = struct.Struct("<8sIIqqq24x")
 
 
+MAX_ORDER = 65536        # the largest n a plan accepts (hipla_plan_create)
+_I64_MAX = np.iinfo(np.int64).max
+
+
 def _sizes(batch):
+    """(totA, totV) of a batch; refuses n outside 1..MAX_ORDER and offsets whose extent would not
+    fit in int64 (compared with the room left, so nothing wraps)."""
     n = np.asarray(batch.n, dtype=np.int64)
     if len(n) == 0:
         return 0, 0
-    if (n < 1).any() or (batch.offA < 0).any() or (batch.offV < 0).any():
-        raise ValueError("every cell needs n >= 1 and non-negative offsets")
-    return int((batch.offA + n * n).max()), int((batch.offV + n).max())
+    offA = np.asarray(batch.offA, dtype=np.int64)
+    offV = np.asarray(batch.offV, dtype=np.int64)
+    if (n < 1).any() or (n > MAX_ORDER).any() or (offA < 0).any() or (offV < 0).any():
+        raise ValueError(f"every cell needs 1 <= n <= {MAX_ORDER} and non-negative offsets")
+    if (offA > _I64_MAX - n * n).any() or (offV > _I64_MAX - n).any():
+        raise ValueError("a cell's offset puts its extent beyond int64")
+    return int((offA + n * n).max()), int((offV + n).max())
+
+
+def _extents_ok(n, offA, offV, totA, totV):
+    """Every cell inside [0, totA) x [0, totV), with no sum that can wrap."""
+    n = n.astype(np.int64)
+    return not ((n < 1).any() or (n > MAX_ORDER).any() or (offA < 0).any() or (offV < 0).any()
+                or (n * n > totA).any() or (offA > totA - n * n).any() or (n > totV).any()
+                or (offV > totV - n).any())
 
 
 def save(batch: RaggedBatch, path: str, with_b: bool = True) -> None:
@@ -76,7 +94,7 @@ def load(path: str) -> RaggedBatch:
     pos += 8 * totA
     B = np.frombuffer(raw, "<f8", totV, pos).astype(np.float64) if flags & 1 else np.zeros(totV)
     batch = RaggedBatch(n, offA, offV, A, B)
-    if _sizes(batch) != (totA, totV) and nc:
+    if nc and (not _extents_ok(n, offA, offV, totA, totV) or _sizes(batch) != (totA, totV)):
         raise ValueError(f"{path}: a cell lies outside the batch")
     return batch
 

@@ -161,3 +161,41 @@ def test_forked_back_substitution_graph_replays(hip, oracle_lib, mode):
                 assert err.max() <= TOL, (graph, rep, err.max())
             torch.empty(1 << 20, device=B.device).normal_()      # eager work between replays
         plan.close()
+
+
+@pytest.mark.parametrize("tuned", [False, True])
+def test_wide_range_batch(hip, oracle_lib, tuned):
+    """The precision workload: rates inside 1e-20..1e4 and 2-norm condition numbers 1e6..1e14 (all
+    seven sizes, 2% of the 10,000-cell batch, seed 3).  Mixed mode untuned (every class on the
+    FP32 path) and autotuned: kept cells within 1e-8 of the FP64 oracle, handed-back cells equal to
+    it, codes equal to the oracle's; FP64 mode equal to the oracle in every cell."""
+    import torch
+    from python_amd.generators import uneven
+    batch = uneven(seed=3, wide=True, scale=0.02)
+    print("cells", batch.ncells, "kappa hist", batch.meta["kappa_hist"], flush=True)
+    Xo, co = oracle_lib.solve_batch(batch.n, batch.offA, batch.offV, batch.A, batch.B, nthreads=8)
+    A, B = hip.to_device(batch)
+    plan = hip.Plan.for_batch(batch, mode="mixed", graph=True)
+    X = torch.zeros_like(B)
+    codes = torch.empty(batch.ncells, dtype=torch.int32, device=B.device)
+    if tuned:
+        plan.autotune(A, B, X, codes)
+    X.zero_()
+    _, _, st = plan.solve(A, B, X, codes, stats=True)
+    torch.cuda.synchronize()
+    Xh, ch, stats = X.cpu().numpy(), codes.cpu().numpy(), hip.stats_to_numpy(st)
+    err = _relerr(batch, Xh, Xo, co)
+    fb = stats["fallback"]
+    print("singular (oracle code 4):", int((co == 4).sum()), "handed back:", int((fb == 1).sum()),
+          "FP64 direct:", int((fb == 2).sum()), "max err of kept cells:", float(err[fb == 0].max(initial=0.0)),
+          flush=True)
+    assert np.all(err <= TOL)
+    for c in np.where((fb != 0) & (co != 4))[0]:
+        o, n = batch.offV[c], batch.n[c]
+        assert np.array_equal(Xh[o:o + n], Xo[o:o + n]), c
+    assert np.array_equal(ch, co), np.where(ch != co)[0][:10]
+    X64, c64, _ = hip.solve_batch(batch, mode="fp64")
+    assert np.array_equal(c64, co)
+    for c in np.where(co != 4)[0]:
+        o, n = batch.offV[c], batch.n[c]
+        assert np.array_equal(X64[o:o + n], Xo[o:o + n]), c

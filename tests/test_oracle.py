@@ -261,3 +261,30 @@ def test_batched_oracle_matches_single(oracle_lib):
         code, x = oracle_lib.solve_matrix(a, b)
         assert code == codes[c]
         assert np.array_equal(x, X[batch.offV[c]:batch.offV[c] + len(b)])
+
+
+def test_batched_inverse_matches_single(oracle_lib):
+    """ref_invert_batch (the C4 CPU baseline) equals invert_matrix cell by cell, bit for bit,
+    on a ragged batch with a singular cell: its output slot stays as the caller left it (zeros),
+    as cpu_invert_matrix leaves inverse_out untouched on error (source/matrix_cpu.c:242-245)."""
+    from python_amd.generators import macro_q, pack
+    rng = np.random.default_rng(11)
+    mats = [macro_q(rng, 1, n)[0] for n in (7, 33, 5, 64, 20)]
+    mats.insert(2, np.array([[1.0, 2.0], [2.0, 4.0]]))
+    batch = pack(mats)
+    Inv, codes = oracle_lib.invert_batch(batch.n, batch.offA, batch.A, nthreads=3)
+    for c in range(batch.ncells):
+        a, _ = batch.cell(c)
+        n = a.shape[0]
+        code, inv = oracle_lib.invert_matrix(a)
+        assert code == codes[c]
+        got = Inv[batch.offA[c]:batch.offA[c] + n * n].reshape(n, n)
+        if code == 0:
+            assert np.array_equal(got, inv)
+        else:
+            assert code == 1 and not got.any()
+    # a row-major / column-major mix-up in the batch loop would transpose a non-symmetric cell
+    a, _ = batch.cell(4)
+    assert a.shape == (64, 64)
+    assert not np.array_equal(Inv[batch.offA[4]:batch.offA[4] + 64 * 64].reshape(64, 64).T,
+                              oracle_lib.invert_matrix(a)[1])

@@ -88,3 +88,39 @@ def test_reference_text_fixture_round_trip(tmp_path):
     p = str(tmp_path / "ion.hrb")
     wire.write_native(b, p)
     _same(wire.load(p), b)
+
+
+def test_offsets_near_int64_max_are_refused(tmp_path):
+    """An offset whose extent offA + n^2 wraps past INT64_MAX must not pass the extent checks of
+    either reader, nor hipla_plan_create (which checks before it touches the device)."""
+    import ctypes
+    from python_amd import HiplaError, lib, wire
+    b = _batch(np.random.default_rng(3))
+    p = str(tmp_path / "w.hrb")
+    wire.save(b, p)
+    raw = open(p, "rb").read()
+    nc = len(b.n)
+    off_pos = 64 + 4 * nc + (-4 * nc) % 8 + 8 * (nc - 1)          # offA of the last cell
+    q = str(tmp_path / "wrap.hrb")
+    open(q, "wb").write(raw[:off_pos] + struct.pack("<q", 2 ** 63 - 2) + raw[off_pos + 8:])
+    with pytest.raises(ValueError):
+        wire.load(q)
+    with pytest.raises(HiplaError):
+        wire.read_native(q)
+    # an order above the plans' limit is refused too
+    n_pos = 64 + 4 * (nc - 1)
+    q2 = str(tmp_path / "big.hrb")
+    open(q2, "wb").write(raw[:n_pos] + struct.pack("<i", 70000) + raw[n_pos + 4:])
+    with pytest.raises(ValueError):
+        wire.load(q2)
+    with pytest.raises(HiplaError):
+        wire.read_native(q2)
+    # the plan refuses the same offsets before any device work
+    n = np.array([3, 4], dtype=np.int32)
+    offA = np.array([0, 2 ** 63 - 2], dtype=np.int64)
+    offV = np.array([0, 3], dtype=np.int64)
+    h = ctypes.c_void_p()
+    rc = lib().hipla_plan_create(ctypes.byref(h), 2, n.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                 offA.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                 offV.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 0, 0)
+    assert rc == 5 and b"cell 1" in lib().hipla_last_error_detail()
