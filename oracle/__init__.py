@@ -27,6 +27,14 @@ _szp = ctypes.POINTER(ctypes.c_size_t)
 _up = ctypes.POINTER(ctypes.c_uint)
 
 
+class RefIonCell(ctypes.Structure):
+    """ref_ion_cell of hipla_ref.h: per-ion tables and coefficients of the n_e iteration."""
+    _fields_ = [("elem", _ip), ("bits", _ip),
+                ("pi", _dp), ("rr", _dp), ("di", _dp), ("dr", _dp), ("qr", _dp), ("cxr", _dp), ("cxi", _dp),
+                ("inner_beg", _ip), ("inner_cnt", _ip), ("inner_rate", _dp), ("inner_yield", _dp),
+                ("has_cx", ctypes.c_int)]
+
+
 def build(force: bool = False) -> str:
     """Compile liboracle.so with gcc (oracle/Makefile)."""
     if force or not os.path.exists(_LIB):
@@ -51,6 +59,12 @@ def lib():
         L.ref_solve_dd_refined.argtypes = [_dp, _dp, ctypes.c_int, _dp, ctypes.c_int]
         L.ref_solve_batch.argtypes = [ctypes.c_int, _ip, _i64p, _i64p, _dp, _dp, _dp, _ip, ctypes.c_int]
         L.ref_invert_batch.argtypes = [ctypes.c_int, _ip, _i64p, _dp, _dp, _ip, ctypes.c_int]
+        L.ref_populate_ion_rate_matrix.argtypes = [ctypes.c_int, ctypes.POINTER(RefIonCell), ctypes.c_double,
+                                                   ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.ref_populate_ion_rate_matrix.restype = None
+        L.ref_ion_populations.argtypes = [ctypes.c_int, ctypes.POINTER(RefIonCell), _dp, _dp, _ip, _ip]
+        L.ref_ion_populations_batch.argtypes = [ctypes.c_int, _ip, _i64p, ctypes.POINTER(RefIonCell), _dp, _dp,
+                                                _ip, _ip, _ip, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -168,3 +182,76 @@ def invert_batch(nsz, offA, A, nthreads: int = 1):
     lib().ref_invert_batch(len(nsz), nsz.ctypes.data_as(_ip), offA.ctypes.data_as(_i64p), _d(A), _d(Inv),
                            codes.ctypes.data_as(_ip), int(nthreads))
     return Inv, codes
+
+
+def _ion_struct(t, keep):
+    """RefIonCell over the arrays of ``t`` (an IonBatch, or a dict with the same fields); the
+    contiguous copies are appended to ``keep`` so they outlive the call."""
+    def arr(v, dtype):
+        a = np.ascontiguousarray(v, dtype=dtype)
+        keep.append(a)
+        return a.ctypes.data_as(_ip if dtype == np.int32 else _dp)
+    get = (lambda k: getattr(t, k)) if not isinstance(t, dict) else (lambda k: t[k])
+    coef = get("coef")
+    return RefIonCell(arr(get("elem"), np.int32), arr(get("bits"), np.int32),
+                      *[arr(coef[k], np.float64) for k in ("pi", "rr", "di", "dr", "qr", "cxr", "cxi")],
+                      arr(get("inner_beg"), np.int32), arr(get("inner_cnt"), np.int32),
+                      arr(get("inner_rate"), np.float64), arr(np.ravel(get("inner_yield")), np.float64),
+                      int(get("has_cx")))
+
+
+def populate_ion_rate_matrix(cell, xne, nh1, nh2):
+    """populate_ion_rate_matrix (source/matrix_ion.c:444-659) of one cell: returns (A, b).
+    ``cell`` is a dict of one cell's per-ion arrays (see ``ion_cell``)."""
+    keep = []
+    st = _ion_struct(cell, keep)
+    n = len(cell["elem"])
+    a = np.zeros((n, n))
+    b = np.zeros(n)
+    lib().ref_populate_ion_rate_matrix(n, ctypes.byref(st), float(xne), float(nh1), float(nh2), _d(a), _d(b))
+    return a, b
+
+
+def ion_populations(cell):
+    """The n_e iteration of matrix_ion_populations (source/matrix_ion.c:219-382) for one cell:
+    returns (status, density, ne, iterations, last solve code); density is the input density when
+    status != 0."""
+    keep = []
+    st = _ion_struct(cell, keep)
+    dens = _c(cell["density"]).copy()
+    ne = ctypes.c_double(0.0)
+    it = ctypes.c_int(0)
+    code = ctypes.c_int(0)
+    status = lib().ref_ion_populations(len(dens), ctypes.byref(st), _d(dens), ctypes.byref(ne), ctypes.byref(it),
+                                       ctypes.byref(code))
+    return status, dens, ne.value, it.value, code.value
+
+
+def ion_cell(batch, c):
+    """One cell of an IonBatch as a dict of its own arrays (inner records re-indexed from 0)."""
+    o, n = int(batch.offV[c]), int(batch.n[c])
+    sl = slice(o, o + n)
+    beg, cnt = batch.inner_beg[sl], batch.inner_cnt[sl]
+    r0 = int(beg[0]) if n else 0
+    r1 = int(beg[-1] + cnt[-1]) if n else 0
+    return {"elem": batch.elem[sl], "bits": batch.bits[sl], "coef": {k: v[sl] for k, v in batch.coef.items()},
+            "inner_beg": (beg - r0).astype(np.int32), "inner_cnt": cnt, "inner_rate": batch.inner_rate[r0:r1],
+            "inner_yield": batch.inner_yield[r0:r1], "density": batch.density[sl], "has_cx": batch.has_cx}
+
+
+def ion_populations_batch(batch, nthreads: int = 1):
+    """The iteration over every cell of an IonBatch: returns (density, ne, status, iters, codes)."""
+    keep = []
+    st = _ion_struct(batch, keep)
+    nsz = np.ascontiguousarray(batch.n, dtype=np.int32)
+    offV = np.ascontiguousarray(batch.offV, dtype=np.int64)
+    dens = _c(batch.density).copy()
+    nc = len(nsz)
+    ne = np.zeros(nc)
+    status = np.zeros(nc, dtype=np.int32)
+    iters = np.zeros(nc, dtype=np.int32)
+    codes = np.zeros(nc, dtype=np.int32)
+    lib().ref_ion_populations_batch(nc, nsz.ctypes.data_as(_ip), offV.ctypes.data_as(_i64p), ctypes.byref(st),
+                                    _d(dens), _d(ne), status.ctypes.data_as(_ip), iters.ctypes.data_as(_ip),
+                                    codes.ctypes.data_as(_ip), int(nthreads))
+    return dens, ne, status, iters, codes

@@ -450,3 +450,185 @@ def macro_inverse(ncells: int = 2000, seed: int = 4, sizes=(256, 512, 1024)) -> 
     batch = pack_groups(groups, order)
     batch.meta = {"workload": "macro_inverse", "seed": seed}
     return batch
+
+
+# ------------------------------------------------------------------------------------------
+# The ionization-balance iteration (source/matrix_ion.c:219-382 around populate_ion_rate_matrix
+# :444-659): per-ion rate coefficients instead of assembled matrices
+# ------------------------------------------------------------------------------------------
+ION_FIRST, ION_MAXST, ION_DI, ION_DR, ION_CXI = 1 << 16, 1 << 17, 1 << 18, 1 << 19, 1 << 20
+ION_MAXJUMP = 10                 # inner-shell electron yields per record (prob[0..9])
+SLACK_Z = 30                     # the one-stage slack element: not hydrogen, no inner shells
+ION_COEF = ("pi", "rr", "di", "dr", "qr", "cxr", "cxi")
+
+
+@dataclass
+class IonBatch:
+    """Per-cell ion tables and rate coefficients of the n_e iteration, ragged over cells.
+
+    Cell c has n[c] ions at offV[c] in every per-ion array: ``elem`` (element id, unique within
+    the cell), ``bits`` (istate | z << 8 | ION_* flags), the coefficients of ``ION_COEF``
+    (pi_rates, rr_rates, di/dr/qrecomb_coeffs, charge-exchange recombination and ionization
+    coefficients), ``inner_beg`` / ``inner_cnt`` (the ion's inner-shell records, global indices
+    into ``inner_rate`` and ``inner_yield`` [nrec, 10]), and ``density`` (absolute ion densities
+    the iteration starts from).  ``offA`` lays out the n x n rate matrices back to back."""
+    n: np.ndarray
+    offV: np.ndarray
+    offA: np.ndarray
+    elem: np.ndarray
+    bits: np.ndarray
+    coef: dict
+    inner_beg: np.ndarray
+    inner_cnt: np.ndarray
+    inner_rate: np.ndarray
+    inner_yield: np.ndarray
+    density: np.ndarray
+    has_cx: int = 1
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def ncells(self) -> int:
+        return int(self.n.shape[0])
+
+    @property
+    def totV(self) -> int:
+        return int(self.offV[-1] + self.n[-1]) if self.ncells else 0
+
+    def subset(self, cells) -> "IonBatch":
+        cells = np.asarray(list(cells), dtype=np.int64)
+        ns = self.n[cells].astype(np.int32)
+        offV = np.zeros(len(cells), dtype=np.int64)
+        offA = np.zeros(len(cells), dtype=np.int64)
+        offV[1:] = np.cumsum(ns.astype(np.int64))[:-1]
+        offA[1:] = np.cumsum(ns.astype(np.int64) ** 2)[:-1]
+        idx = np.concatenate([np.arange(self.offV[c], self.offV[c] + self.n[c]) for c in cells])
+        beg, cnt = self.inner_beg[idx], self.inner_cnt[idx]
+        recs = np.concatenate([np.arange(b, b + k) for b, k in zip(beg, cnt)]).astype(np.int64) \
+            if cnt.sum() else np.zeros(0, dtype=np.int64)
+        nbeg = np.zeros(len(idx), dtype=np.int32)
+        nbeg[1:] = np.cumsum(cnt)[:-1]
+        return IonBatch(ns, offV, offA, self.elem[idx].copy(), self.bits[idx].copy(),
+                        {k: v[idx].copy() for k, v in self.coef.items()}, nbeg, cnt.copy(),
+                        self.inner_rate[recs].copy(), self.inner_yield[recs].copy(), self.density[idx].copy(),
+                        self.has_cx, dict(self.meta))
+
+
+def _ion_table(n):
+    """(elem, z, istate, first, last) per ion of the element layout of order n."""
+    blocks = element_layout(n)
+    elem, z, ist, first, last = [], [], [], [], []
+    for i, (s, m, ab) in enumerate(blocks):
+        zz = SLACK_Z if i == len(blocks) - 1 else ELEMENTS[i % len(ELEMENTS)][1] - 1
+        for k in range(m):
+            elem.append(i)
+            z.append(zz)
+            ist.append(k + 1)
+            first.append(k == 0)
+            last.append(k == m - 1)
+    return (np.array(elem), np.array(z), np.array(ist), np.array(first), np.array(last), blocks)
+
+
+def ion_iteration_cells(rng, ncells: int, n: int, nh_range=(1e6, 1e12)):
+    """Rate coefficients of ncells cells with n ions each (element layout of ``element_layout``).
+
+    Photoionization rates log-uniform over 1e-8..1e1 s^-1, radiative / dielectronic recombination
+    coefficients over 1e-14..1e-10 cm^3 s^-1, three-body over 1e-30..1e-26 cm^6 s^-1, collisional
+    ionization over 1e-14..1e-9, charge exchange over 1e-12..1e-9 (recombination on every ion
+    above hydrogen that is not a neutral; ionization on a fifth of the metal ions below their top
+    stage); 80 % of ions carry a collisional-ionization rate and 70 % a dielectronic one.  Ions
+    with n_elec = min(z - istate + 1, 11) >= 3 whose Auger jumps stay inside their element get
+    one inner-shell record (rate over 1e-10..1e-2, yields summing to one; a tenth get a second
+    record).  Hydrogen density log-uniform over nh_range, element densities nh * abundance,
+    initial stage fractions a Dirichlet draw.  Returns a dict of per-ion arrays [ncells, n] and
+    the records."""
+    elem, z, ist, first, last, blocks = _ion_table(n)
+    ab = np.array([blocks[e][2] for e in elem])
+    bits = ist | (z << 8) | np.where(first, ION_FIRST, 0) | np.where(last, ION_MAXST, 0)
+    bits = np.broadcast_to(bits, (ncells, n)).copy()
+    bits |= np.where(rng.random((ncells, n)) < 0.8, ION_DI, 0)
+    bits |= np.where(rng.random((ncells, n)) < 0.7, ION_DR, 0)
+    metal_mid = (z != 1) & (z != SLACK_Z) & ~last
+    bits |= np.where(metal_mid[None, :] & (rng.random((ncells, n)) < 0.2), ION_CXI, 0)
+    coef = {
+        "pi": _logu(rng, 1e-8, 1e1, (ncells, n)),
+        "rr": _logu(rng, 1e-14, 1e-10, (ncells, n)),
+        "di": _logu(rng, 1e-14, 1e-9, (ncells, n)),
+        "dr": _logu(rng, 1e-14, 1e-10, (ncells, n)),
+        "qr": _logu(rng, 1e-30, 1e-26, (ncells, n)),
+        "cxr": np.where((z != 1) & ~first, 1.0, 0.0)[None, :] * _logu(rng, 1e-12, 1e-9, (ncells, n)),
+        "cxi": np.where(bits & ION_CXI, 1.0, 0.0) * _logu(rng, 1e-12, 1e-9, (ncells, n)),
+    }
+    # inner-shell records: jumps d = 1 .. n_elec - 1 must stay inside the ion's element
+    block_end = np.array([blocks[e][0] + blocks[e][1] - 1 for e in elem])
+    n_elec = np.minimum(z - ist + 1, 11)
+    ok = (n_elec >= 3) & (np.arange(n) + n_elec - 1 <= block_end) & (z != SLACK_Z)
+    nrec = np.where(ok[None, :], 1 + (rng.random((ncells, n)) < 0.1), 0).astype(np.int32)
+    tot = int(nrec.sum())
+    rate = _logu(rng, 1e-10, 1e-2, tot)
+    yld = np.zeros((tot, ION_MAXJUMP))
+    owner = np.repeat(np.tile(np.arange(n), ncells), nrec.ravel())
+    for j in range(tot):
+        k = n_elec[owner[j]] - 1
+        y = rng.gamma(1.0, 1.0, k)
+        yld[j, :k] = y / y.sum()
+    nh = _logu(rng, *nh_range, ncells)
+    frac = np.zeros((ncells, n))
+    for (s, m, _ab) in blocks:
+        g = rng.gamma(1.0, 1.0, (ncells, m))
+        frac[:, s:s + m] = g / g.sum(axis=1, keepdims=True)
+    density = nh[:, None] * ab[None, :] * frac
+    return {"elem": np.broadcast_to(elem, (ncells, n)).astype(np.int32), "bits": bits.astype(np.int32),
+            "coef": coef, "nrec": nrec, "inner_rate": rate, "inner_yield": yld, "density": density, "nh": nh}
+
+
+def ion_iteration_batch(ncells: int = 10000, seed: int = 6, scale: float = 1.0, sizes=UNEVEN_SIZES,
+                        counts=UNEVEN_COUNTS) -> IonBatch:
+    """The lock-step n_e iteration workload: the uneven size mix (n in {12..256}, counts
+    3000/2500/1500/1000/1000/500/500 for 10,000 cells, shuffled), coefficients from
+    ``ion_iteration_cells``, seed 6.  ``scale`` shrinks every count."""
+    rng = np.random.default_rng(seed)
+    counts = [max(1, int(round(c * scale * ncells / 10000))) for c in counts]
+    order = np.concatenate([np.full(c, n, dtype=np.int32) for n, c in zip(sizes, counts)])
+    rng.shuffle(order)
+    per = {n: ion_iteration_cells(rng, c, n) for n, c in zip(sizes, counts)}
+    return _pack_ion(per, order, {"workload": "ion_iteration", "seed": seed, "counts": dict(zip(sizes, counts))})
+
+
+def _pack_ion(per, order, meta) -> IonBatch:
+    ns = np.asarray(order, dtype=np.int32)
+    nc = len(ns)
+    offV = np.zeros(nc, dtype=np.int64)
+    offA = np.zeros(nc, dtype=np.int64)
+    offV[1:] = np.cumsum(ns.astype(np.int64))[:-1]
+    offA[1:] = np.cumsum(ns.astype(np.int64) ** 2)[:-1]
+    totV = int(ns.astype(np.int64).sum())
+    elem = np.empty(totV, dtype=np.int32)
+    bits = np.empty(totV, dtype=np.int32)
+    coef = {k: np.empty(totV) for k in ION_COEF}
+    density = np.empty(totV)
+    cnt = np.empty(totV, dtype=np.int32)
+    rates, ylds = [], []
+    cursor = {n: 0 for n in per}
+    # each size's record list is split per cell in the order its cells are taken
+    rec_start = {n: np.concatenate([[0], np.cumsum(per[n]["nrec"].sum(axis=1))]) for n in per}
+    for c, n in enumerate(ns):
+        n = int(n)
+        d = per[n]
+        i = cursor[n]
+        cursor[n] = i + 1
+        sl = slice(offV[c], offV[c] + n)
+        elem[sl] = d["elem"][i]
+        bits[sl] = d["bits"][i]
+        for k in ION_COEF:
+            coef[k][sl] = d["coef"][k][i]
+        density[sl] = d["density"][i]
+        cnt[sl] = d["nrec"][i]
+        r0, r1 = rec_start[n][i], rec_start[n][i + 1]
+        rates.append(d["inner_rate"][r0:r1])
+        ylds.append(d["inner_yield"][r0:r1])
+    beg = np.zeros(totV, dtype=np.int32)
+    beg[1:] = np.cumsum(cnt)[:-1]
+    return IonBatch(ns, offV, offA, elem, bits, coef, beg, cnt,
+                    np.concatenate(rates) if rates else np.zeros(0),
+                    np.concatenate(ylds).reshape(-1, ION_MAXJUMP) if ylds else np.zeros((0, ION_MAXJUMP)),
+                    density, 1, meta)
