@@ -68,7 +68,8 @@ def measure_roofline(plan, A, B, reps=10):
         k["flops"] += float(g["flops"])
         k["launches"] += 1
         k["classes"].append(f"n<={g['nmax']} x{g['count']}")
-        classes.append({"kernel": name, "nmax": g["nmax"], "count": g["count"], "ms": round(float(t), 4)})
+        classes.append({"kernel": name, "nmax": g["nmax"], "count": g["count"], "ms": round(float(t), 4),
+                        "bytes": float(g["bytes"]), "flops": float(g["flops"])})
     dom = max(by_kernel, key=lambda nm: by_kernel[nm]["ms"])
     dk = by_kernel[dom]
     bw = dk["bytes"] / (dk["ms"] * 1e-3) / 1e9
@@ -229,8 +230,12 @@ def bench_c3(args):
 
 
 def bench_c4(args):
-    """Macro-atom inversion: 2,000 cells N = I - Q, n in {256, 512, 1024} (thirds), seed 4; FP64
-    inverse in GSL's LU order; relative Frobenius error against the oracle on a sample of cells."""
+    """Macro-atom inversion: 2,000 cells N = I - Q, n in {256, 512, 1024} (thirds), seed 4.  Two
+    plans on the same cells: FP64 (blocked Gauss-Jordan sweep with FP64 MFMA block products) and
+    mixed (the sweep in FP32 with split BF16x3 MFMA products, then two FP64 refinement steps);
+    ``value`` is the faster one.  Relative Frobenius error against the oracle on two cells per
+    order.  Roofline: algorithmic flops 2 n^3 per inverse over the step time, against the FP64
+    peak (both plans' dominant work is FP64: the sweep's block products, or the residual GEMMs)."""
     import torch
     import oracle
     import python_amd as pa
@@ -239,32 +244,47 @@ def bench_c4(args):
     batch = macro_inverse(ncells=200 if args.quick else 2000)
     print(f"[bench c4] generated {batch.ncells} cells in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     A, _ = pa.to_device(batch)
-    plan = pa.Plan.for_batch(batch, mode="fp64", graph=False)
-    Inv = torch.empty_like(A)
-    codes = torch.empty(batch.ncells, dtype=torch.int32, device=A.device)
-    steps = max(1, min(args.steps, 3))
-    ms = _timed_ms(lambda: plan.invert(A, Inv, codes), steps, 1)
     flops = float(sum(2.0 * n ** 3 for n in batch.n))
     byts = float(sum(16.0 * n * n for n in batch.n))
-    Ih, ch = Inv.cpu().numpy(), codes.cpu().numpy()
     oracle.build()
-    errs = {}
+    refs = {}
     for nsz in np.unique(batch.n):
         for c in np.where(batch.n == nsz)[0][:2]:
             a, _ = batch.cell(c)
-            code, ref = oracle.invert_matrix(a)
+            refs[int(c)] = oracle.invert_matrix(a)[1]
+    modes = {}
+    Inv = torch.empty_like(A)
+    for mode in ("fp64", "mixed"):
+        plan = pa.Plan.for_batch(batch, mode=mode, graph=True)
+        codes = torch.empty(batch.ncells, dtype=torch.int32, device=A.device)
+        steps = max(1, min(args.steps, 5))
+        ms = _timed_ms(lambda: plan.invert(A, Inv, codes), steps, 1)
+        Ih, ch = Inv.cpu().numpy(), codes.cpu().numpy()
+        errs = {}
+        for c, ref in refs.items():
+            nsz = int(batch.n[c])
             got = Ih[batch.offA[c]:batch.offA[c] + nsz * nsz].reshape(nsz, nsz)
-            errs.setdefault(int(nsz), []).append(float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
+            errs.setdefault(nsz, []).append(float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
+        tf = flops / (ms * 1e-3) / 1e12
+        modes[mode] = {"inverses_per_s": round(batch.ncells / (ms * 1e-3), 2), "ms_per_step": round(ms, 3),
+                       "steps": steps, "codes_nonzero": int((ch != 0).sum()), "rel_frobenius_err_sample": errs,
+                       "max_rel_frobenius_err_sample": max(max(v) for v in errs.values()),
+                       "fp64_equivalent_tflops": round(tf, 3)}
+        print(f"[bench c4] {mode}: {ms:.2f} ms", file=sys.stderr, flush=True)
+        plan.close()
+    best = min(modes, key=lambda m: modes[m]["ms_per_step"])
+    ms = modes[best]["ms_per_step"]
     tf = flops / (ms * 1e-3) / 1e12
     return _emit({
         "metric": "inverses/s, macro-atom matrices N = I - Q, n in {256,512,1024}", "unit": "inverses/s",
-        "value": round(batch.ncells / (ms * 1e-3), 2), "ms_per_step": round(ms, 3), "steps": steps, "n_gpus": 1,
-        "dtype": "f64", "data": "synthetic (seeded substochastic Q, seed 4)",
-        "config": {"workload": f"macro_inverse: {batch.ncells} cells, n in {{256,512,1024}} thirds"},
-        "codes_nonzero": int((ch != 0).sum()),
-        "rel_frobenius_err_sample": errs, "max_rel_frobenius_err_sample": max(max(v) for v in errs.values()),
+        "value": modes[best]["inverses_per_s"], "ms_per_step": ms, "n_gpus": 1,
+        "dtype": "f64" if best == "fp64" else "bf16x3+f64", "data": "synthetic (seeded substochastic Q, seed 4)",
+        "config": {"workload": f"macro_inverse: {batch.ncells} cells, n in {{256,512,1024}} thirds", "plan": best},
+        "modes": modes,
         "roofline": {"bound": "fp64", "achieved": round(tf, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(tf / FP64_PEAK_TFLOPS, 5), "hbm_achieved_gbs": round(byts / (ms * 1e-3) / 1e9, 2)}})
+                     "frac": round(tf / FP64_PEAK_TFLOPS, 5), "traffic": None,
+                     "kernel": "whole inverse (Gauss-Jordan sweep on the matrix cores)",
+                     "hbm_achieved_gbs": round(byts / (ms * 1e-3) / 1e9, 2)}})
 
 
 def bench_c5(args):
@@ -310,6 +330,64 @@ def bench_c5(args):
         "gradient_bound": "1e-8 |g| + 1e-10 sum|lambda||dA||x| against the oracle on 200 sampled cells"})
 
 
+def bench_ion(args):
+    """The ionization balance of 10,000 cells in lock step (the uneven order mix 12..256, seed 6):
+    one step = the whole n_e fixed point of every cell (hipla_ion_populations: GPU assembly of the
+    rate matrices, FP64 GSL-order solves of the cells still iterating, floors / damping /
+    convergence on the GPU) from the same starting densities.  Checked bit for bit against the
+    oracle's per-cell restatement of matrix_ion_populations, which also gives the CPU baseline
+    (one worker per host core)."""
+    import torch
+    import oracle
+    from python_amd.generators import ion_iteration_batch
+    from python_amd.ion import IonIteration
+    t0 = time.time()
+    batch = ion_iteration_batch(scale=0.1 if args.quick else 1.0)
+    print(f"[bench ion] generated {batch.ncells} cells in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    it = IonIteration(batch)
+    steps = max(1, min(args.steps, 10))
+    ms = _timed_ms(lambda: it.populations(), steps, max(1, args.warmup))
+    dens, ne, st, iters, codes = (t.cpu().numpy() for t in it.populations())
+    oracle.build()
+    cores = _cores()
+    t0 = time.perf_counter()
+    wd, wne, wst, wit, wcodes = oracle.ion_populations_batch(batch, nthreads=cores)
+    t_cpu = time.perf_counter() - t0
+    reps = 1
+    while t_cpu < 10.0 and reps < 50:
+        t1 = time.perf_counter()
+        oracle.ion_populations_batch(batch, nthreads=cores)
+        t_cpu += time.perf_counter() - t1
+        reps += 1
+    ok = wst == 0
+    equal = bool(np.array_equal(st, wst) and np.array_equal(iters, wit) and np.array_equal(codes, wcodes)
+                 and np.array_equal(ne[ok], wne[ok]) and np.array_equal(dens.view(np.uint64), wd.view(np.uint64)))
+    # solves per cell: iterations + 1 for a converged cell, the iteration count for MAXITERATIONS,
+    # iterations + 1 for a singular one
+    nsolve = np.where((wst != 0) & (wit == 200), 200, wit + 1)
+    n = batch.n.astype(np.float64)
+    solve_bytes = float((nsolve * 8.0 * (n * n + 2 * n)).sum())
+    # assembly: the n x n matrix and b written, the nine per-ion arrays read, per iteration
+    asm_bytes = float((nsolve * 8.0 * (n * n + n + 9 * n)).sum())
+    byts = solve_bytes + asm_bytes
+    bw = byts / (ms * 1e-3) / 1e9
+    return _emit({
+        "metric": "ionization balances/s, lock-step n_e iteration (matrix_ion_populations) on 10k uneven cells",
+        "unit": "cells/s", "value": round(batch.ncells / (ms * 1e-3), 1), "ms_per_step": round(ms, 3), "steps": steps,
+        "n_gpus": 1, "dtype": "f64", "data": "synthetic (seeded per-ion rate coefficients, seed 6)",
+        "config": {"workload": f"ion_iteration: {batch.ncells} cells, n in {{12..256}}, iterations to convergence"},
+        "equal_to_oracle": equal, "solves_per_s": round(float(nsolve.sum()) / (ms * 1e-3), 1),
+        "solves_per_step": int(nsolve.sum()), "iterations_max": int(wit.max()),
+        "cells_failed": int((wst != 0).sum()), "iterations_hist": np.bincount(wit, minlength=1).tolist()[:30],
+        "roofline": {"bound": "hbm", "achieved": round(bw, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(bw / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "whole iteration (assembly + FP64 solves + update), one step",
+                     "algorithmic_bytes_per_step": round(byts)},
+        "cpu_baseline": {"value": round(reps * batch.ncells / t_cpu, 1), "unit": "cells/s", "cores": cores,
+                         "kind": "port", "sample": f"the whole batch x {reps} ({t_cpu:.1f} s, one OpenMP worker "
+                                                   "per core)"}})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -320,10 +398,11 @@ def main():
     ap.add_argument("--quick", action="store_true", help="1/10 of the batch (development only)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="plan set-up, graph timing and the dominant-kernel measurement only (profiler runs)")
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "ion"],
                     help="c1: 900 x n = 30; c2: the headline batch (default); c3: wide-range rates, "
                          "refinement statistics; "
-                         "c4: macro-atom inversion; c5: T_e gradient (forward solve + adjoint)")
+                         "c4: macro-atom inversion; c5: T_e gradient (forward solve + adjoint); "
+                         "ion: lock-step n_e iteration of the ionization balance")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs on a single MI355X (the cell batch would shard "
@@ -336,7 +415,7 @@ def main():
     from python_amd.generators import uneven
 
     if args.config != "c2":
-        return {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}[args.config](args)
+        return {"c1": bench_c1, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5, "ion": bench_ion}[args.config](args)
 
     t_gen = time.time()
     batch = uneven(scale=0.1 if args.quick else 1.0)
@@ -465,6 +544,10 @@ def main():
         "codes_equal_oracle": bool(np.array_equal(cm_h, co)),
         "fallback_rate": round(float((fb == 1).sum()) / batch.ncells, 4),
         "fp64_direct_cells": int((fb == 2).sum()),
+        "value_runs": ("the batched FP64 GSL-order kernels: the autotuner timed FP32 LU + FP64 refinement "
+                       "against FP64 per size class and routed every class to FP64 (FP32 rounding of these "
+                       "rate matrices drops their small rates; most FP32 factorisations are handed back)")
+        if int((fb == 2).sum()) == batch.ncells else "mixed FP32 LU + FP64 refinement on the tuned classes",
         "refine_steps_hist": steps_hist,
         "batched_fp64": None if ms_fp64 is None else
         {"solves_per_s": round(batch.ncells / (ms_fp64 * 1e-3), 1), "ms_per_step": round(ms_fp64, 4),

@@ -105,9 +105,15 @@ int hipla_plan_set_refinement (hipla_plan plan, int max_refine, double tol);
 int hipla_solve (hipla_plan plan, const double *dA, const double *dB, double *dX, int *dcodes,
                  hipla_cell_stats *dstats, void *stream);
 
-/* Invert every cell (offV is ignored); dInv has A's layout.  dcodes gets invert_matrix's codes
- * per cell.  Every mode runs the FP64 inverse (GSL's LU order, within 1e-8 relative Frobenius of
- * the reference's gsl_linalg_LU_invx); dstats is not written. */
+/* Invert every cell (offV is ignored); dInv has A's layout and must not overlap dA.  dcodes gets
+ * invert_matrix's codes per cell (0, or 1 = GSL_EDOM for an exact zero pivot); dstats is not
+ * written.  Cells whose order is a multiple of 64 from 128 on and whose matrix is row diagonally
+ * dominant (as N = I - Q of the macro-atom scheme is) are swept by a blocked Gauss-Jordan
+ * inverse on the matrix cores: in HIPLA_MODE_FP64 with FP64 MFMA block products, in
+ * HIPLA_MODE_MIXED with split BF16x3 MFMA products in FP32 followed by two FP64 residual
+ * refinement steps.  Every other cell runs the GSL-order FP64 LU inverse with partial pivoting.
+ * All are within 1e-8 relative Frobenius of the reference's gsl_linalg_LU_invx on the
+ * workloads tested; none is in its operation order. */
 int hipla_invert (hipla_plan plan, const double *dA, double *dInv, int *dcodes, hipla_cell_stats *dstats,
                   void *stream);
 
@@ -142,6 +148,55 @@ int hipla_batch_write (const char *path, int ncells, const int *n, const int64_t
                        const double *A, const double *B);
 int hipla_batch_info (const char *path, int *ncells, int64_t *totA, int64_t *totV, int *has_b);
 int hipla_batch_read (const char *path, int *n, int64_t *offA, int64_t *offV, double *A, double *B);
+
+/* ------------------------------------------------------------------------------------------ *
+ * The ionization balance of many cells in lock step.  Restates, for a batch of cells at once,  *
+ * the n_e fixed point of matrix_ion_populations (source/matrix_ion.c:219-382: DENSITY_MIN      *
+ * floors, xnew, the FRACTIONAL_ERROR = 0.03 test, (xnew + xne) / 2 damping, MAXITERATIONS =    *
+ * 200) with the rate matrix assembled on the GPU as populate_ion_rate_matrix does (:444-659)   *
+ * and solved in FP64 in GSL's order (solve_matrix, :276).  In the reference the per-ion         *
+ * coefficients (pi_rates, rr_rates, di/dr/qrecomb_coeffs, charge exchange, inner-shell rates   *
+ * and yields) are computed from atomic data before the loop (:98-183) and are constants of the  *
+ * cell inside it; here the caller supplies them.                                               *
+ * ------------------------------------------------------------------------------------------ */
+#define HIPLA_ION_MAXN 512            /* ions per cell                                          */
+/* per-ion bits: istate (1 = neutral) in bits 0-7, z in bits 8-15, and: */
+#define HIPLA_ION_FIRST (1 << 16)     /* first ion of its element (ele[].firstion)              */
+#define HIPLA_ION_MAXST (1 << 17)     /* highest stage of its element (ele[].istate_max)        */
+#define HIPLA_ION_DI (1 << 18)        /* has a collisional-ionization rate (dere_di_flag > 0)    */
+#define HIPLA_ION_DR (1 << 19)        /* has a dielectronic-recombination rate (drflag > 0)      */
+#define HIPLA_ION_CXI (1 << 20)       /* has a charge-exchange ionization rate (by H II)         */
+#define HIPLA_ION_MAXJUMP 10          /* inner-shell electron yields per record (prob[0..9])      */
+
+/* Device pointers; per-ion arrays in B's layout (cell c's ions at offV[c]).  elem is an element
+ * id that is unique within a cell (the reference compares ion[].z).  Inner-shell record k of
+ * ion i (inner_beg[i] <= k < inner_beg[i] + inner_cnt[i], global record indices) ionizes i with
+ * rate inner_rate[k] into the stages i + d, d = 1 .. min(z - istate + 1, 11) - 1, with yields
+ * inner_yield[10 k + d - 1].  has_cx: n_charge_exchange > 0 (charge-exchange recombination by
+ * H I on every ion above hydrogen, coefficient cxr). */
+typedef struct
+{
+  const int32_t *elem, *bits;
+  const double *pi, *rr, *di, *dr, *qr, *cxr, *cxi;
+  const int32_t *inner_beg, *inner_cnt;
+  const double *inner_rate, *inner_yield;
+  int has_cx;
+} hipla_ion_rates;
+
+typedef struct hipla_ion_s *hipla_ion;
+
+/* A context for ncells cells of n[c] ions (1 <= n[c] <= HIPLA_ION_MAXN); host arrays.  0 or 5. */
+int hipla_ion_create (hipla_ion *ctx, int ncells, const int *n, const int64_t *offV);
+int hipla_ion_destroy (hipla_ion ctx);
+
+/* Run every cell's iteration to its end.  dDensity (B's layout) holds the absolute ion densities
+ * to start from (xplasma->density) and receives the new ones for cells that converge; dNe
+ * (ncells) gets get_ne of the new densities, dStatus matrix_ion_populations' return value (0, or
+ * -1 for a singular solve or MAXITERATIONS, density then untouched), dIters the iteration count
+ * at the end, dCodes the last solve_matrix code.  Synchronous: it reads the number of cells still
+ * iterating back after every few iterations and returns once none is left.  0 or 5. */
+int hipla_ion_populations (hipla_ion ctx, const hipla_ion_rates *rates, double *dDensity, double *dNe, int *dStatus,
+                           int *dIters, int *dCodes, void *stream);
 
 /* Time the kernel variants for this plan's size histogram and keep the fastest per size class
  * (the choice is cached per process and, if HIPLA_TUNE_CACHE names a file, across processes).

@@ -34,6 +34,12 @@ class CellStats(ctypes.Structure):
 
 CELL_STATS_BYTES = ctypes.sizeof(CellStats)
 
+
+class IonRates(ctypes.Structure):
+    """hipla_ion_rates (include/hipla.h): device pointers of the n_e iteration's per-ion tables."""
+    _fields_ = [(k, c_void_p) for k in ("elem", "bits", "pi", "rr", "di", "dr", "qr", "cxr", "cxi", "inner_beg",
+                                        "inner_cnt", "inner_rate", "inner_yield")] + [("has_cx", c_int)]
+
 # name -> (restype, argtypes); every function declared in include/hipla.h
 SIGNATURES = {
     "solve_matrix": (c_int, [_dp, _dp, c_int, _dp, c_int]),
@@ -53,6 +59,10 @@ SIGNATURES = {
     "hipla_batch_write": (c_int, [c_char_p, c_int, _ip, _i64p, _i64p, _dp, _dp]),
     "hipla_batch_info": (c_int, [c_char_p, _ip, _i64p, _i64p, _ip]),
     "hipla_batch_read": (c_int, [c_char_p, _ip, _i64p, _i64p, _dp, _dp]),
+    "hipla_ion_create": (c_int, [ctypes.POINTER(c_void_p), c_int, _ip, _i64p]),
+    "hipla_ion_destroy": (c_int, [c_void_p]),
+    "hipla_ion_populations": (c_int, [c_void_p, ctypes.POINTER(IonRates), c_void_p, c_void_p, c_void_p, c_void_p,
+                                      c_void_p, c_void_p]),
     "hipla_plan_autotune": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "hipla_plan_describe": (c_int, [c_void_p, ctypes.c_char_p, c_int]),
     "hipla_plan_time_groups": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, _dp, c_int]),

@@ -30,3 +30,40 @@ int tile_f64_limit ();
 hipError_t launch_tile_f64 (const SolveArgs &a, int nmax, bool keep, int grid, hipStream_t st);
 void tile_kernel_symbol (int nmax, bool keep, char *buf, size_t len);
 void fp64_kernel_symbol (int nmax, bool keep, char *buf, size_t len);
+
+// ---- FP64 MFMA Gauss-Jordan inverse of row-diagonally dominant cells (k_gj.hip) ----
+struct GjArgs
+{
+  const int64_t *offA;
+  const int *list;      // the cells of one size class
+  int ncells;           // in list
+  int nsz;              // their order (a multiple of 64, >= 128)
+  int sb;               // diagonal block of the step
+  const double *A;      // input
+  double *M;            // output, inverted in place
+  int *codes;
+  int *flag;            // per cell: 0 swept here, 1 handed to k_invert_f64, 2 singular
+  int *fb;              // [count, cells...] handed to k_invert_f64
+};
+bool gj_eligible (int n);
+hipError_t gj_reset (int *fb, hipStream_t st);
+hipError_t gj_enqueue_class (const GjArgs &g, hipStream_t st);
+
+// ---- mixed-precision inverse: BF16x3 MFMA sweep in FP32 + FP64 refinement (k_gjx.hip) ----
+struct GjxArgs
+{
+  const int64_t *offA;
+  const int *list;
+  int ncells;
+  int nsz;
+  int sb;
+  const double *A;      // input (M = A read column-major)
+  float *Mf;            // FP32 sweep store: X0 ~ M^-1 after the sweep
+  float *Rf;            // FP32 residual store
+  double *X;            // output: the refined inverse
+  int *codes;
+  int *flag;
+  int *fb;
+};
+#define GX_REFINE_STEPS 2   // FP64 residual refinement steps of the mixed inverse
+hipError_t gjx_enqueue_class (const GjxArgs &g, int refine_steps, hipStream_t st);

@@ -90,6 +90,13 @@ struct hipla_plan_s
   int fb_route_used[5] = { 0, 0, 0, 0, 0 };
   int *d_list_desc = nullptr;  // every cell, largest order first (inverse launches)
   double *d_inv_work = nullptr; // inverse: per-workgroup LU workspace
+  // inverse: size classes swept by the MFMA Gauss-Jordan kernels (n, first, count into d_gj_list),
+  // the cells left to the pivoting kernel, per-cell flags and the hand-off list
+  std::vector<int> gj_n, gj_first, gj_count;
+  int *d_gj_list = nullptr, *d_gj_flag = nullptr, *d_gj_fb = nullptr, *d_inv_rest = nullptr;
+  int inv_nrest = 0, inv_rest_nmax = 0;
+  float *d_gx_Mf = nullptr, *d_gx_Rf = nullptr;   // mixed inverse: FP32 sweep and residual stores (A's layout)
+  bool inv_ready = false;
   int64_t inv_wstride = 0;
   int inv_slots = 0;
   double *d_LU = nullptr;     // keep-factors store (A's layout)
@@ -113,6 +120,9 @@ struct hipla_plan_s
   const double *last_A = nullptr;
   int solved = 0;
 };
+
+// the FP64 kernel shape (KernelKind, param) an order n is routed to
+void plan_route_fp64 (int n, int *kind, int *param);
 
 // launch every group of a solve / invert on `st`, one after another
 int plan_enqueue_solve (hipla_plan p, const double *dA, const double *dB, double *dX, int *dcodes,

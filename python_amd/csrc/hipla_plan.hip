@@ -71,6 +71,11 @@ static void route_fp64 (int n, int &kind, int &param)
   }
 }
 
+void plan_route_fp64 (int n, int *kind, int *param)
+{
+  route_fp64 (n, *kind, *param);
+}
+
 static int free_plan_device (hipla_plan p)
 {
   if (p->gexec)
@@ -80,7 +85,8 @@ static int free_plan_device (hipla_plan p)
   p->gexec = nullptr;
   p->graph = nullptr;
   void *bufs[] = { p->d_n,   p->d_offA, p->d_offV, p->d_list, p->d_list_desc, p->d_inv_work, p->d_fallback,
-                   p->d_LU,  p->d_piv,  p->own_F ? p->d_F : nullptr, p->d_Y, p->d_lam };
+                   p->d_LU,  p->d_piv,  p->own_F ? p->d_F : nullptr, p->d_Y, p->d_lam,
+                   p->d_gj_list, p->d_gj_flag, p->d_gj_fb, p->d_inv_rest, p->d_gx_Mf, p->d_gx_Rf };
   for (void *b : bufs)
     if (b)
       (void) hipla_dev_free (b);

@@ -302,10 +302,66 @@ hipError_t launch_invert_f64 (const SolveArgs &a, int nmax, int grid, double *wo
   return hipGetLastError ();
 }
 
+// Route the cells once: orders the MFMA Gauss-Jordan sweep takes (multiples of 64 from 128 on)
+// in size classes, every other cell to the GSL-order pivoting kernel; the workspace of that
+// kernel is sized for the largest order it may receive (cells of the sweep classes it is handed
+// back included).
 int plan_invert_prepare (hipla_plan p)
 {
   const DeviceInfo &dev = hipla_device_info ();
-  const int nmax = *std::max_element (p->n.begin (), p->n.end ());
+  if (!p->inv_ready)
+  {
+    std::vector<int> order (p->ncells), rest;
+    for (int c = 0; c < p->ncells; c++)
+      order[c] = c;
+    std::stable_sort (order.begin (), order.end (), [&] (int a, int b) { return p->n[a] > p->n[b]; });
+    std::vector<int> gl;
+    for (int i = 0; i < p->ncells;)
+    {
+      int j = i;
+      while (j < p->ncells && p->n[order[j]] == p->n[order[i]])
+        j++;
+      const int nn = p->n[order[i]];
+      if (gj_eligible (nn))
+      {
+        p->gj_n.push_back (nn);
+        p->gj_first.push_back ((int) gl.size ());
+        p->gj_count.push_back (j - i);
+        for (int k = i; k < j; k++)
+          gl.push_back (order[k]);
+      }
+      else
+        for (int k = i; k < j; k++)
+          rest.push_back (order[k]);
+      i = j;
+    }
+    p->inv_nrest = (int) rest.size ();
+    for (int c : rest)
+      p->inv_rest_nmax = std::max (p->inv_rest_nmax, p->n[c]);
+    if (!gl.empty ())
+    {
+      HIPLA_CHECK (hipla_dev_alloc (&p->d_gj_list, sizeof (int) * gl.size ()));
+      HIPLA_CHECK (hipMemcpy (p->d_gj_list, gl.data (), sizeof (int) * gl.size (), hipMemcpyHostToDevice));
+      HIPLA_CHECK (hipla_dev_alloc (&p->d_gj_flag, sizeof (int) * p->ncells));
+      HIPLA_CHECK (hipla_dev_alloc (&p->d_gj_fb, sizeof (int) * (p->ncells + 1)));
+    }
+    if (!gl.empty () && p->mode == HIPLA_MODE_MIXED)
+    {
+      HIPLA_CHECK (hipla_dev_alloc (&p->d_gx_Mf, sizeof (float) * p->totA));
+      HIPLA_CHECK (hipla_dev_alloc (&p->d_gx_Rf, sizeof (float) * p->totA));
+    }
+    if (!rest.empty ())
+    {
+      HIPLA_CHECK (hipla_dev_alloc (&p->d_inv_rest, sizeof (int) * rest.size ()));
+      HIPLA_CHECK (hipMemcpy (p->d_inv_rest, rest.data (), sizeof (int) * rest.size (), hipMemcpyHostToDevice));
+    }
+    p->inv_ready = true;
+  }
+  // the pivoting kernel's largest order: every cell when some sweep class exists (any of its
+  // cells may be handed back), else the largest of the rest
+  const int nmax = p->gj_n.empty () ? p->inv_rest_nmax : *std::max_element (p->n.begin (), p->n.end ());
+  if (nmax <= 0)
+    return 0;
   if (invert_f64_smem (nmax) > dev.max_smem - 1024)
   {
     hipla_set_detail ("hipla_invert: n = %d exceeds the inverse kernel's panel capacity", nmax);
@@ -329,12 +385,12 @@ int plan_enqueue_invert (hipla_plan p, const double *dA, double *dInv, int *dcod
                          hipStream_t st)
 {
   (void) dstats;
-  if (!p->d_inv_work)
+  if (!p->inv_ready || (!p->d_inv_work && (p->inv_nrest > 0 || !p->gj_n.empty ())))
   {
     hipla_set_detail ("hipla_invert: workspace not prepared");
     return 5;
   }
-  const int nmax = *std::max_element (p->n.begin (), p->n.end ());
+  const int nmax_all = *std::max_element (p->n.begin (), p->n.end ());
   SolveArgs a;
   a.n = p->d_n;
   a.offA = p->d_offA;
@@ -343,14 +399,61 @@ int plan_enqueue_invert (hipla_plan p, const double *dA, double *dInv, int *dcod
   a.B = nullptr;
   a.X = nullptr;
   a.codes = dcodes;
-  a.list = p->d_list_desc;   // every cell, largest first
-  a.nlist = p->ncells;
   a.LU = nullptr;
   a.piv = nullptr;
   a.dcount = nullptr;
   a.F = p->d_F;
   a.Y = p->d_Y;
-  HIPLA_CHECK (launch_invert_f64 (a, nmax, p->inv_slots, p->d_inv_work, p->inv_wstride, dInv, st));
+  if (!p->gj_n.empty ())
+  {
+    HIPLA_CHECK (gj_reset (p->d_gj_fb, st));
+    for (size_t k = 0; k < p->gj_n.size () && p->mode == HIPLA_MODE_MIXED; k++)
+    {
+      GjxArgs g;
+      g.offA = p->d_offA;
+      g.list = p->d_gj_list + p->gj_first[k];
+      g.ncells = p->gj_count[k];
+      g.nsz = p->gj_n[k];
+      g.sb = 0;
+      g.A = dA;
+      g.Mf = p->d_gx_Mf;
+      g.Rf = p->d_gx_Rf;
+      g.X = dInv;
+      g.codes = dcodes;
+      g.flag = p->d_gj_flag;
+      g.fb = p->d_gj_fb;
+      HIPLA_CHECK (gjx_enqueue_class (g, GX_REFINE_STEPS, st));
+    }
+    for (size_t k = 0; k < p->gj_n.size () && p->mode != HIPLA_MODE_MIXED; k++)
+    {
+      GjArgs g;
+      g.offA = p->d_offA;
+      g.list = p->d_gj_list + p->gj_first[k];
+      g.ncells = p->gj_count[k];
+      g.nsz = p->gj_n[k];
+      g.sb = 0;
+      g.A = dA;
+      g.M = dInv;
+      g.codes = dcodes;
+      g.flag = p->d_gj_flag;
+      g.fb = p->d_gj_fb;
+      HIPLA_CHECK (gj_enqueue_class (g, st));
+    }
+  }
+  if (p->inv_nrest > 0)
+  {
+    a.list = p->d_inv_rest;   // largest first
+    a.nlist = p->inv_nrest;
+    HIPLA_CHECK (launch_invert_f64 (a, p->inv_rest_nmax, p->inv_slots, p->d_inv_work, p->inv_wstride, dInv, st));
+  }
+  if (!p->gj_n.empty ())
+  {
+    // cells the sweep handed back (not diagonally dominant): a persistent grid over the list
+    a.list = p->d_gj_fb + 1;
+    a.nlist = p->ncells;
+    a.dcount = p->d_gj_fb;
+    HIPLA_CHECK (launch_invert_f64 (a, nmax_all, p->inv_slots, p->d_inv_work, p->inv_wstride, dInv, st));
+  }
   return 0;
 }
 

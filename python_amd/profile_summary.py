@@ -136,7 +136,46 @@ def main(out=None):
           f"algorithmic ({traffic / roof['algorithmic_bytes_per_launch']:.2f}x)"]
     with open(os.path.join(ROOT, "profiles", f"{short}_uneven10k_hbm.txt"), "w") as f:
         f.write("\n".join(hl) + "\n")
-    print("\n".join(lines + [""] + hl))
+    cl = _class_table(rl, trace, fetch, write, ff, fw, is_part)
+    with open(os.path.join(ROOT, "profiles", "size_classes_uneven10k.txt"), "w") as f:
+        f.write("\n".join(cl) + "\n")
+    print("\n".join(lines + [""] + hl + [""] + cl))
+
+
+def _class_table(rl, trace, fetch, write, ff, fw, is_part):
+    """Every FP64 size class of the bench workload: rocprofv3 average duration of its kernels,
+    algorithmic bytes, achieved GB/s and the calibrated FETCH_SIZE + WRITE_SIZE bytes."""
+    out = ["# Size classes of the bench workload (uneven 10k-cell batch), each launched in isolation",
+           f"command: {CMD}", "",
+           "Per class: rocprofv3 average duration of each kernel of the class (factor, then back "
+           "substitution where there is one), algorithmic bytes 8 (n^2 + 2 n) per cell, the achieved "
+           "rate on those bytes, and the HBM bytes the counters saw (FETCH_SIZE / 0.5 + WRITE_SIZE, per "
+           "dispatch).", "",
+           f"{'kernel(s)':58s} {'cells':>6s} {'us':>9s} {'alg MB':>8s} {'GB/s':>8s} {'% HBM':>6s} "
+           f"{'HBM MB':>8s} {'x alg':>6s}"]
+    for c in rl.get("classes", []):
+        parts = [x.strip() for x in c["kernel"].split(" + ")]
+        us = 0.0
+        hbm = 0.0
+        for part in parts:
+            d = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in trace if is_part(r["Kernel_Name"], part)]
+            # classes sharing a kernel instantiation: the dispatches of this class's count
+            same = [x for x in rl["classes"] if x["kernel"] == c["kernel"]]
+            if len(same) > 1:
+                d = d[same.index(c)::len(same)]
+            us += sum(d) / len(d) / 1e3 if d else float("nan")
+            fk = [v for k, v in fetch.items() if is_part(k, part)]
+            wk = [v for k, v in write.items() if is_part(k, part)]
+            if fk and wk:
+                fv, wv = fk[0], wk[0]
+                if len(same) > 1:
+                    fv, wv = fv[same.index(c)::len(same)], wv[same.index(c)::len(same)]
+                hbm += (sum(fv) / len(fv)) * 1024.0 / ff + (sum(wv) / len(wv)) * 1024.0 / fw
+        alg = c.get("bytes", 0.0)
+        gbs = alg / (us * 1e-6) / 1e9 if us > 0 else float("nan")
+        out.append(f"{c['kernel'][:58]:58s} {c['count']:6d} {us:9.1f} {alg / 1e6:8.2f} {gbs:8.1f} "
+                   f"{100 * gbs / 8000:6.2f} {hbm / 1e6:8.2f} {hbm / alg if alg else float('nan'):6.2f}")
+    return out
 
 
 if __name__ == "__main__":
