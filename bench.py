@@ -354,7 +354,7 @@ def bench_ion(args):
     wd, wne, wst, wit, wcodes = oracle.ion_populations_batch(batch, nthreads=cores)
     t_cpu = time.perf_counter() - t0
     reps = 1
-    while t_cpu < 10.0 and reps < 50:
+    while t_cpu < 10.0 and reps < 50 and not args.no_cpu:
         t1 = time.perf_counter()
         oracle.ion_populations_batch(batch, nthreads=cores)
         t_cpu += time.perf_counter() - t1

@@ -66,32 +66,45 @@ struct IonArgs
 };
 
 // ---- initial state: element densities, fractions, get_ne (source/matrix_ion.c:84-96,129,208) ----
+// The ordered sums run on one lane over values staged in LDS (formed in parallel, each rounded
+// exactly as the reference rounds it), so only the additions are serial.
 __global__ __launch_bounds__ (64) void k_ion_init (IonArgs s)
 {
+  __shared__ int s_el[HIPLA_ION_MAXN];
+  __shared__ double s_d[HIPLA_ION_MAXN];
   const int lane = threadIdx.x;
   for (int cell = blockIdx.x; cell < s.ncells; cell += gridDim.x)
   {
     const int n = s.n[cell];
     const int64_t o = s.offV[cell];
-    const int *el = s.r.elem + o;
     const double *d = s.density_in + o;
+    const int *bt = s.r.bits + o;
+    for (int i = lane; i < n; i += 64)
+    {
+      s_el[i] = s.r.elem[o + i];
+      s_d[i] = d[i];
+    }
+    __syncthreads ();
     for (int i = lane; i < n; i += 64)
     {
       // elem_dens[z] accumulated over the element's ions in ascending order
-      const int e = el[i];
+      const int e = s_el[i];
       double ed = 0.0;
       for (int j = 0; j < n; j++)
-        if (el[j] == e)
-          ed = ed + d[j];
+        if (s_el[j] == e)
+          ed = ed + s_d[j];
       s.elem_dens[o + i] = ed;
-      s.newden[o + i] = d[i] / ed;
+      s.newden[o + i] = s_d[i] / ed;
     }
+    __syncthreads ();
+    for (int i = lane; i < n; i += 64)
+      s_d[i] = s_d[i] * (ION_ISTATE (bt[i]) - 1);
+    __syncthreads ();
     if (lane == 0)
     {
-      const int *bt = s.r.bits + o;
       double ne = 0;
       for (int k = 0; k < n; k++)
-        ne += d[k] * (ION_ISTATE (bt[k]) - 1);
+        ne += s_d[k];
       s.xne[cell] = ne;
       s.iter[cell] = 0;
       s.active[cell] = 1;
@@ -99,6 +112,7 @@ __global__ __launch_bounds__ (64) void k_ion_init (IonArgs s)
       s.iters[cell] = 0;
       s.codes_out[cell] = 0;
     }
+    __syncthreads ();
   }
 }
 
@@ -249,6 +263,8 @@ __global__ __launch_bounds__ (ION_ASM_THREADS) void k_ion_assemble (IonArgs s)
 // ---- the fixed-point step of matrix_ion_populations (source/matrix_ion.c:276-381) ----
 __global__ __launch_bounds__ (64) void k_ion_update (IonArgs s)
 {
+  __shared__ double s_p[HIPLA_ION_MAXN];
+  __shared__ int s_out;
   const int lane = threadIdx.x;
   for (int cell = blockIdx.x; cell < s.ncells; cell += gridDim.x)
   {
@@ -272,30 +288,27 @@ __global__ __launch_bounds__ (64) void k_ion_update (IonArgs s)
       }
       continue;
     }
-    // new fractions with the DENSITY_MIN floor (:307-336)
+    // new fractions with the DENSITY_MIN floor (:307-336) and the terms of xnew (:341-346)
     for (int i = lane; i < n; i += 64)
     {
       double v = x[i];
       if (v < ION_DENSITY_MIN)
         v = ION_DENSITY_MIN;
       s.newden[o + i] = v;
+      s_p[i] = v * (ION_ISTATE (bt[i]) - 1) * ed[i];
     }
-    int outcome = 0;   // 0 continue, 1 converged, 2 MAXITERATIONS
+    __syncthreads ();
     if (lane == 0)
     {
-      // xnew in the reference's order (:341-349), from the floored fractions
+      // xnew summed in the reference's order
       double xnew = 0.0;
       for (int k = 0; k < n; k++)
-      {
-        double v = x[k];
-        if (v < ION_DENSITY_MIN)
-          v = ION_DENSITY_MIN;
-        xnew += v * (ION_ISTATE (bt[k]) - 1) * ed[k];
-      }
+        xnew += s_p[k];
       if (xnew < ION_DENSITY_MIN)
         xnew = ION_DENSITY_MIN;
       const double xne = s.xne[cell];
       s.codes_out[cell] = code;
+      int outcome;   // 0 continue, 1 converged, 2 MAXITERATIONS
       if (fabs ((xne - xnew) / (xnew)) < ION_FRACTIONAL_ERROR || xnew < 1.e-6)
         outcome = 1;
       else
@@ -305,8 +318,10 @@ __global__ __launch_bounds__ (64) void k_ion_update (IonArgs s)
         s.iter[cell] = it;
         outcome = it == ION_MAXITERATIONS ? 2 : 0;
       }
+      s_out = outcome;
     }
-    outcome = __shfl (outcome, 0);
+    __syncthreads ();
+    const int outcome = s_out;
     if (outcome == 1)
     {
       // absolute densities and get_ne of them (:366-378)
@@ -315,18 +330,16 @@ __global__ __launch_bounds__ (64) void k_ion_update (IonArgs s)
         double v = x[i];
         if (v < ION_DENSITY_MIN)
           v = ION_DENSITY_MIN;
-        s.density[o + i] = v * ed[i];
+        const double dn = v * ed[i];
+        s.density[o + i] = dn;
+        s_p[i] = dn * (ION_ISTATE (bt[i]) - 1);
       }
+      __syncthreads ();
       if (lane == 0)
       {
         double ne = 0;
         for (int k = 0; k < n; k++)
-        {
-          double v = x[k];
-          if (v < ION_DENSITY_MIN)
-            v = ION_DENSITY_MIN;
-          ne += (v * ed[k]) * (ION_ISTATE (bt[k]) - 1);
-        }
+          ne += s_p[k];
         s.ne[cell] = ne;
         s.status[cell] = 0;
         s.iters[cell] = s.iter[cell];
@@ -344,6 +357,7 @@ __global__ __launch_bounds__ (64) void k_ion_update (IonArgs s)
       else
         atomicAdd (s.nactive, 1);
     }
+    __syncthreads ();
   }
 }
 
@@ -524,6 +538,7 @@ static int ion_enqueue_iteration (hipla_ion c, const IonArgs &a, hipStream_t st)
   {
     if (!c->route_used[r])
       continue;
+    hipStream_t rst = st;
     SolveArgs sa;
     sa.n = a.n;
     sa.offA = a.offA;
@@ -540,12 +555,12 @@ static int ion_enqueue_iteration (hipla_ion c, const IonArgs &a, hipStream_t st)
     sa.F = c->plan->d_F;
     sa.Y = c->plan->d_Y;
     if (r < 3)
-      HIPLA_CHECK (launch_wave_f64 (sa, r == 0 ? 16 : r == 1 ? 32 : 64, false, 2 * dev.num_cu, st));
+      HIPLA_CHECK (launch_wave_f64 (sa, r == 0 ? 16 : r == 1 ? 32 : 64, false, 2 * dev.num_cu, rst));
     else if (r == 3)
-      HIPLA_CHECK (launch_wgr_f64 (sa, c->route_nmax[3], false, dev.num_cu, st));
+      HIPLA_CHECK (launch_wgr_f64 (sa, c->route_nmax[3], false, dev.num_cu, rst));
     else
       HIPLA_CHECK (launch_blocked_f64 (sa, c->route_nmax[4], blocked_f64_wgs_per_cu (c->route_nmax[4]) * dev.num_cu,
-                                       false, st));
+                                       false, rst));
   }
   const int ugrid = std::max (1, std::min (c->ncells, 8 * dev.num_cu));
   hipLaunchKernelGGL (k_ion_update, dim3 (ugrid), dim3 (64), 0, st, a);

@@ -12,7 +12,8 @@
 //   4. M[O,S] <- -M[O,S] D             k_gj_tile<5>
 // After the last step M holds M^-1.  Every product is a 64 x 64 x 64 block product on
 // v_mfma_f64_16x16x4f64 (four waves, a 32 x 32 quarter each, operands staged through LDS in two
-// K halves).  The sweep does not pivot: it runs on cells whose A is row diagonally dominant
+// K halves; lane l holds A[l % 16][l / 16] and B[l / 16][l % 16] of a 16 x 4 x 16 step and
+// element e of its result at row l / 16 + 4 e, column l % 16).  The sweep does not pivot: it runs on cells whose A is row diagonally dominant
 // (|a_ii| >= sum_j!=i |a_ij|, which N = I - Q with substochastic Q is).  M is then column
 // diagonally dominant, every Schur complement stays so, and partial pivoting would never swap a
 // row; k_gj_prepare hands every other cell to the GSL-order pivoting kernel (k_invert_f64).  An
@@ -203,10 +204,9 @@ __global__ __launch_bounds__ (256, 2) void k_gj_tile (GjArgs g)
     {
       if (MODE == 4)
       {
-        const double *src = m + (size_t) (tj * GJ_NB + cw + 16 * bq + lr) * n + ti * GJ_NB + rw + 16 * bi + 4 * lk;
-        const double2 x0 = *reinterpret_cast<const double2 *> (src);
-        const double2 x1 = *reinterpret_cast<const double2 *> (src + 2);
-        acc[bi][bq] = gj_d4{ x0.x, x0.y, x1.x, x1.y };
+        // v_mfma_f64_16x16x4f64 keeps element e of lane l at row l/16 + 4 e, column l % 16
+        const double *src = m + (size_t) (tj * GJ_NB + cw + 16 * bq + lr) * n + ti * GJ_NB + rw + 16 * bi + lk;
+        acc[bi][bq] = gj_d4{ src[0], src[4], src[8], src[12] };
       }
       else
         acc[bi][bq] = gj_d4{ 0.0, 0.0, 0.0, 0.0 };
@@ -251,9 +251,10 @@ __global__ __launch_bounds__ (256, 2) void k_gj_tile (GjArgs g)
 #pragma unroll
     for (int bq = 0; bq < 2; bq++)
     {
-      double *dst = m + (size_t) (tj * GJ_NB + cw + 16 * bq + lr) * n + ti * GJ_NB + rw + 16 * bi + 4 * lk;
-      *reinterpret_cast<double2 *> (dst) = make_double2 (acc[bi][bq][0], acc[bi][bq][1]);
-      *reinterpret_cast<double2 *> (dst + 2) = make_double2 (acc[bi][bq][2], acc[bi][bq][3]);
+      double *dst = m + (size_t) (tj * GJ_NB + cw + 16 * bq + lr) * n + ti * GJ_NB + rw + 16 * bi + lk;
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+        dst[4 * e] = acc[bi][bq][e];
     }
 }
 
@@ -263,13 +264,12 @@ __global__ void k_gj_reset (int *fb)
     fb[0] = 0;
 }
 
-// the sweep runs only with HIPLA_INVERT_SWEEP=1 in the environment; otherwise every cell takes
-// the GSL-order pivoting kernel
+// HIPLA_INVERT_SWEEP=0 in the environment sends every cell to the GSL-order pivoting kernel
 bool gj_eligible (int n)
 {
   static const bool on = [] {
     const char *e = getenv ("HIPLA_INVERT_SWEEP");
-    return e && e[0] == '1';
+    return !(e && e[0] == '0');
   }();
   return on && n >= 2 * GJ_NB && n % GJ_NB == 0;
 }

@@ -288,7 +288,9 @@ __global__ __launch_bounds__ (256, 2) void k_gjx_resid (GjxArgs g)
 #pragma unroll
       for (int e = 0; e < 4; e++)
       {
-        const int r = ti * GX_NB + rw + 16 * bi + 4 * lk + e, c = tj * GX_NB + cw + 16 * bq + lr;
+        // v_mfma_f64_16x16x4f64: element e of lane l at row l/16 + 4 e (the FP32-output BF16
+        // instruction keeps it at row 4 (l/16) + e)
+        const int r = ti * GX_NB + rw + 16 * bi + lk + 4 * e, c = tj * GX_NB + cw + 16 * bq + lr;
         acc[bi][bq][e] = r == c ? 1.0 : 0.0;
       }
   for (int kb = 0; kb < n; kb += GX_KC)
@@ -326,9 +328,12 @@ __global__ __launch_bounds__ (256, 2) void k_gjx_resid (GjxArgs g)
   for (int bi = 0; bi < 2; bi++)
 #pragma unroll
     for (int bq = 0; bq < 2; bq++)
-      *reinterpret_cast<float4 *> (R + (size_t) (tj * GX_NB + cw + 16 * bq + lr) * n + ti * GX_NB + rw + 16 * bi +
-                                   4 * lk) = make_float4 ((float) acc[bi][bq][0], (float) acc[bi][bq][1],
-                                                          (float) acc[bi][bq][2], (float) acc[bi][bq][3]);
+    {
+      float *dst = R + (size_t) (tj * GX_NB + cw + 16 * bq + lr) * n + ti * GX_NB + rw + 16 * bi + lk;
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+        dst[4 * e] = (float) acc[bi][bq][e];
+    }
 }
 
 // ---- X += X0 R (BF16x3, K = n), the addition in FP64 ----
