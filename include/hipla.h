@@ -56,6 +56,16 @@ const char *hipla_last_error_detail (void);
  * NULL.  Returns 0. */
 int hipla_alloc_stats (int64_t *device_allocations, int64_t *device_bytes_live);
 
+/* No counterpart in the reference: solve_matrix for callers that cannot allocate device memory
+ * (code under hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * computation and codes as solve_matrix (FP64, GSL's order, codes 0/2/3/4 in *dcode; x not
+ * written on 4), but on DEVICE pointers dA (row-major n x n), dB, dX (n) and dcode (one int),
+ * asynchronous on `stream`, and with no allocation of any kind: the cell descriptors and the
+ * factor store live in a static device array of 8.4 MB that each GPU holds until the process
+ * ends.  1 <= n <= 1024, else 5.  Every call on a GPU uses that GPU's one array: calls that
+ * overlap on one GPU (other streams, other threads) race; serialise them. */
+int hipla_solve_matrix_noalloc (const double *dA, const double *dB, int n, double *dX, int *dcode, void *stream);
+
 /* ------------------------------------------------------------------------------------------ *
  * Batched ragged API (new; the reference solves one matrix per call).  It batches the per-cell *
  * loops that call the path: wind_update's cell loop (source/wind_updates2d.c:89-153 ->        *

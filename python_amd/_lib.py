@@ -49,6 +49,7 @@ SIGNATURES = {
     "hipla_finalize": (c_int, []),
     "hipla_last_error_detail": (c_char_p, []),
     "hipla_alloc_stats": (c_int, [_i64p, _i64p]),
+    "hipla_solve_matrix_noalloc": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "hipla_plan_create": (c_int, [ctypes.POINTER(c_void_p), c_int, _ip, _i64p, _i64p, c_int, c_int]),
     "hipla_plan_destroy": (c_int, [c_void_p]),
     "hipla_plan_set_refinement": (c_int, [c_void_p, c_int, c_double]),
