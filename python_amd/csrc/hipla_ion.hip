@@ -5,7 +5,8 @@
 // The reference runs the fixed point one cell at a time and, under CUDA, pays a device
 // allocation and a host<->device copy of the whole matrix per iteration (source/matrix_gpu.cu:
 // 263-307).  Here the per-ion coefficients stay in HBM; one iteration is four launches over all
-// cells plus the FP64 GSL-order solves of the cells still iterating:
+// cells plus the FP64 GSL-order solves of the cells still iterating (one branch per kernel route,
+// side by side):
 //   k_ion_reset     clears the per-route cell lists and the active count (a kernel, so the
 //                   iteration can live in a hipGraph)
 //   k_ion_assemble  one workgroup per active cell: nh1 / nh2 from the current fractions, the
@@ -378,6 +379,11 @@ struct hipla_ion_s
   int route_used[ION_ROUTES] = { 0, 0, 0, 0, 0 };
   int route_nmax[ION_ROUTES] = { 0, 0, 0, 0, 0 };
   hipStream_t cap = nullptr;
+  // the FP64 routes of one iteration run side by side, one stream each, forked after the assembly
+  // and joined before the update (in the graph, parallel branches): once few cells are left, an
+  // iteration costs its slowest route instead of the sum of all five
+  hipStream_t rs[ION_ROUTES] = { nullptr, nullptr, nullptr, nullptr, nullptr };
+  hipEvent_t ev_fork = nullptr, ev_join[ION_ROUTES] = { nullptr, nullptr, nullptr, nullptr, nullptr };
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   IonArgs gargs{};             // the arguments the graph was captured with
@@ -404,6 +410,15 @@ static void ion_free (hipla_ion c)
     (void) hipHostFree (c->h_nactive);
   if (c->cap)
     (void) hipStreamDestroy (c->cap);
+  for (int r = 0; r < ION_ROUTES; r++)
+  {
+    if (c->rs[r])
+      (void) hipStreamDestroy (c->rs[r]);
+    if (c->ev_join[r])
+      (void) hipEventDestroy (c->ev_join[r]);
+  }
+  if (c->ev_fork)
+    (void) hipEventDestroy (c->ev_fork);
   if (c->plan)
     (void) hipla_plan_destroy (c->plan);
 }
@@ -474,6 +489,16 @@ extern "C" int hipla_ion_create (hipla_ion *out, int ncells, const int *n, const
   ICHECK (hipMemcpy (c->d_route, route.data (), sizeof (int) * ncells, hipMemcpyHostToDevice));
   ICHECK (hipStreamCreateWithFlags (&c->cap, hipStreamNonBlocking));
   hipla_count_resource ();
+  ICHECK (hipEventCreateWithFlags (&c->ev_fork, hipEventDisableTiming));
+  hipla_count_resource ();
+  for (int r = 0; r < ION_ROUTES; r++)
+    if (c->route_used[r])
+    {
+      ICHECK (hipStreamCreateWithFlags (&c->rs[r], hipStreamNonBlocking));
+      ICHECK (hipEventCreateWithFlags (&c->ev_join[r], hipEventDisableTiming));
+      hipla_count_resource ();
+      hipla_count_resource ();
+    }
 #undef ICHECK
   *out = c;
   return 0;
@@ -534,11 +559,13 @@ static int ion_enqueue_iteration (hipla_ion c, const IonArgs &a, hipStream_t st)
   const int agrid = std::max (1, std::min (c->ncells, 4 * dev.num_cu));
   hipLaunchKernelGGL (k_ion_assemble, dim3 (agrid), dim3 (ION_ASM_THREADS), ion_asm_smem (nmax), st, a);
   HIPLA_CHECK (hipGetLastError ());
+  HIPLA_CHECK (hipEventRecord (c->ev_fork, st));
   for (int r = 0; r < ION_ROUTES; r++)
   {
     if (!c->route_used[r])
       continue;
-    hipStream_t rst = st;
+    hipStream_t rst = c->rs[r];
+    HIPLA_CHECK (hipStreamWaitEvent (rst, c->ev_fork, 0));
     SolveArgs sa;
     sa.n = a.n;
     sa.offA = a.offA;
@@ -561,6 +588,8 @@ static int ion_enqueue_iteration (hipla_ion c, const IonArgs &a, hipStream_t st)
     else
       HIPLA_CHECK (launch_blocked_f64 (sa, c->route_nmax[4], blocked_f64_wgs_per_cu (c->route_nmax[4]) * dev.num_cu,
                                        false, rst));
+    HIPLA_CHECK (hipEventRecord (c->ev_join[r], rst));
+    HIPLA_CHECK (hipStreamWaitEvent (st, c->ev_join[r], 0));
   }
   const int ugrid = std::max (1, std::min (c->ncells, 8 * dev.num_cu));
   hipLaunchKernelGGL (k_ion_update, dim3 (ugrid), dim3 (64), 0, st, a);
