@@ -38,6 +38,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
 
 
 def _cores():
@@ -266,7 +267,20 @@ def bench_c4(args):
             got = Ih[batch.offA[c]:batch.offA[c] + nsz * nsz].reshape(nsz, nsz)
             errs.setdefault(nsz, []).append(float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
         tf = flops / (ms * 1e-3) / 1e12
+        # MFMA work the plan issues (64 x 64 x 64 block products: nb (nb^2 - 1) per sweep of an order
+        # n = 64 nb; mixed adds two refinement steps of nb^2 blocks with K = n: one FP64 residual,
+        # one BF16x3 correction); BF16x3 issues three BF16 products per block product
+        blk = 2.0 * 64 ** 3
+        sweep = float(sum((n // 64) * ((n // 64) ** 2 - 1) * blk for n in batch.n))
+        refine = float(sum(2 * (n // 64) ** 2 * 2.0 * 64 * 64 * n for n in batch.n))
+        f64_mfma = sweep if mode == "fp64" else refine
+        bf16_mfma = 0.0 if mode == "fp64" else 3.0 * (sweep + refine)
+        mfma = {"fp64_mfma_tflops": round(f64_mfma / (ms * 1e-3) / 1e12, 3),
+                "fp64_mfma_frac": round(f64_mfma / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 5),
+                "bf16_mfma_tflops": round(bf16_mfma / (ms * 1e-3) / 1e12, 3),
+                "bf16_mfma_frac": round(bf16_mfma / (ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 5)}
         modes[mode] = {"inverses_per_s": round(batch.ncells / (ms * 1e-3), 2), "ms_per_step": round(ms, 3),
+                       "mfma_whole_step": mfma,
                        "steps": steps, "codes_nonzero": int((ch != 0).sum()), "rel_frobenius_err_sample": errs,
                        "max_rel_frobenius_err_sample": max(max(v) for v in errs.values()),
                        "fp64_equivalent_tflops": round(tf, 3)}

@@ -157,8 +157,11 @@ __device__ __forceinline__ int xcd_item (int bid, int G)
 // ---- steps 2-4: one 64 x 64 block product per workgroup ----
 //   MODE 2: C(s, tj) = D C(s, tj)           MODE 4: C(ti, tj) -= M(ti, s) M(s, tj)
 //   MODE 5: C(ti, s) = -C(ti, s) D
+// The two K halves of the operands pass through registers: the second half's global loads are
+// in flight while the first runs on the matrix cores.  Four workgroups share a CU (35 KB of LDS
+// and under 128 VGPRs each), so one's loads overlap the others' products.
 template <int MODE>
-__global__ __launch_bounds__ (256, 2) void k_gj_tile (GjArgs g)
+__global__ __launch_bounds__ (256, 4) void k_gj_tile (GjArgs g)
 {
   const int nb = g.nsz / GJ_NB;
   const int tpc = MODE == 4 ? (nb - 1) * (nb - 1) : nb - 1;
@@ -196,6 +199,19 @@ __global__ __launch_bounds__ (256, 2) void k_gj_tile (GjArgs g)
   const int rw = (wv >> 1) * 32, cw = (wv & 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
 
+  // this thread's eight elements of each operand half: A (k = idx >> 6, row = idx & 63) and
+  // B (col = idx >> 5, k = idx & 31) for idx = tid + 256 e
+  const double *abase = m + (size_t) (sb * GJ_NB) * n + ai * GJ_NB;
+  const double *bbase = m + (size_t) (bj * GJ_NB) * n + sb * GJ_NB;
+  double ra[8], rb[8];
+#pragma unroll
+  for (int e = 0; e < 8; e++)
+  {
+    const int ia = tid + 256 * e, ib = tid + 256 * e;
+    ra[e] = abase[(size_t) (ia >> 6) * n + (ia & 63)];
+    rb[e] = bbase[(size_t) (ib >> 5) * n + (ib & 31)];
+  }
+
   gj_d4 acc[2][2];
 #pragma unroll
   for (int bi = 0; bi < 2; bi++)
@@ -212,21 +228,28 @@ __global__ __launch_bounds__ (256, 2) void k_gj_tile (GjArgs g)
         acc[bi][bq] = gj_d4{ 0.0, 0.0, 0.0, 0.0 };
     }
 
+#pragma unroll
   for (int kh = 0; kh < GJ_NB; kh += GJ_KC)
   {
-    // left operand rows ai*64 .. +63, columns (k) sb*64 + kh .. +31
-    for (int idx = tid; idx < GJ_KC * GJ_NB; idx += 256)
+#pragma unroll
+    for (int e = 0; e < 8; e++)
     {
-      const int k = idx >> 6, r = idx & 63;
-      As[k][r] = sgn * m[(size_t) (sb * GJ_NB + kh + k) * n + ai * GJ_NB + r];
-    }
-    // right operand rows (k) sb*64 + kh .. +31, columns bj*64 .. +63
-    for (int idx = tid; idx < GJ_KC * GJ_NB; idx += 256)
-    {
-      const int c = idx >> 5, k = idx & 31;
-      Bs[c][k] = m[(size_t) (bj * GJ_NB + c) * n + sb * GJ_NB + kh + k];
+      const int ia = tid + 256 * e, ib = tid + 256 * e;
+      As[ia >> 6][ia & 63] = sgn * ra[e];
+      Bs[ib >> 5][ib & 31] = rb[e];
     }
     __syncthreads ();
+    if (kh == 0)
+    {
+      // the second half's loads, in flight during the first half's products
+#pragma unroll
+      for (int e = 0; e < 8; e++)
+      {
+        const int ia = tid + 256 * e, ib = tid + 256 * e;
+        ra[e] = abase[(size_t) (GJ_KC + (ia >> 6)) * n + (ia & 63)];
+        rb[e] = bbase[(size_t) (ib >> 5) * n + GJ_KC + (ib & 31)];
+      }
+    }
 #pragma unroll
     for (int k = 0; k < GJ_KC; k += 4)
     {

@@ -178,5 +178,74 @@ def _class_table(rl, trace, fetch, write, ff, fw, is_part):
     return out
 
 
+C4_CMD = ("python_amd/profile_c4.sh  (rocprofv3 --kernel-trace --stats --output-format csv -- python3 bench.py "
+          "--config c4 --steps 1; then rocprofv3 --pmc SQ_VALU_MFMA_BUSY_CYCLES GRBM_GUI_ACTIVE "
+          "SQ_INSTS_VALU_MFMA_MOPS_F64 SQ_INSTS_VALU_MFMA_MOPS_BF16 -- python3 bench.py --config c4 --steps 1 --quick)")
+SIMD_NUM = 1024          # 256 CUs x 4 SIMDs
+
+
+def main_c4(out=None):
+    """profiles/gj_sweep_c4_trace.txt (per-kernel statistics of both plans over one warm-up and one
+    timed inverse each) and profiles/gj_sweep_c4_mfma.txt (per kernel: MFMA busy cycles over
+    GRBM_GUI_ACTIVE x 1024 SIMDs, and the MFMA flops the MOPS counters count)."""
+    out = out or os.path.join(ROOT, "gpurun_out", "prof_c4")
+    stats = list(csv.DictReader(open(os.path.join(out, "trace", "c4_kernel_stats.csv"))))
+    bench = None
+    with open(os.path.join(out, "trace", "..", "trace.log")) as f:
+        for line in f:
+            if line.startswith('{"metric"'):
+                bench = json.loads(line)
+    lines = ["# Kernel trace of the macro-atom inverse (2,000 cells N = I - Q, n in {256, 512, 1024})",
+             f"command: {C4_CMD}", "",
+             "Each plan (fp64: FP64 MFMA sweep; mixed: BF16x3 sweep + FP64 refinement) ran one warm-up and "
+             "one timed inverse: per-batch kernel time = total / 2.", "",
+             f"{'kernel':52s} {'calls':>6s} {'avg us':>10s} {'total ms':>10s} {'per batch ms':>12s} {'%':>6s}"]
+    for r in stats[:18]:
+        if r["Name"].startswith("__amd"):
+            continue
+        lines.append(f"{r['Name'][:52]:52s} {int(r['Calls']):6d} {float(r['AverageNs']) / 1e3:10.1f} "
+                     f"{float(r['TotalDurationNs']) / 1e6:10.2f} {float(r['TotalDurationNs']) / 2e6:12.2f} "
+                     f"{float(r['Percentage']):6.2f}")
+    if bench:
+        for mode, d in bench["modes"].items():
+            lines.append(f"bench.py {mode}: {d['ms_per_step']} ms per batch (HIP events), {d['inverses_per_s']} "
+                         f"inverses/s, {d['fp64_equivalent_tflops']} TFLOP/s of 2 n^3 per inverse")
+    with open(os.path.join(ROOT, "profiles", "gj_sweep_c4_trace.txt"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    per, dur, seen = {}, {}, set()
+    for r in csv.DictReader(open(os.path.join(out, "mfma", "c4_counter_collection.csv"))):
+        k = r["Kernel_Name"][:52]
+        d = per.setdefault(k, {})
+        d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        if r["Dispatch_Id"] not in seen:
+            seen.add(r["Dispatch_Id"])
+            dur[k] = dur.get(k, 0.0) + (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9
+            d["dispatches"] = d.get("dispatches", 0) + 1
+    ml = ["# MFMA counters of the macro-atom inverse kernels (a tenth of the C4 batch: 200 cells, both plans)",
+          f"command: {C4_CMD}", "",
+          "GRBM_GUI_ACTIVE per dispatch is the sum over the 8 XCDs (it reads 8 x clock x duration), so",
+          "MFMA busy = SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE / 8 x 1024 SIMDs); MFMA flop rates =",
+          "MOPS x 512 over the dispatches' durations, against the dense peaks 78.6 TFLOP/s (FP64) and",
+          "2500 TFLOP/s (BF16).", "",
+          f"{'kernel':52s} {'disp':>5s} {'ms':>8s} {'MFMA busy %':>11s} {'F64 TF/s':>9s} {'% F64':>6s} "
+          f"{'BF16 TF/s':>9s} {'% BF16':>7s}"]
+    for k, d in sorted(per.items(), key=lambda kv: -kv[1].get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0)):
+        busy = d.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0)
+        if busy == 0:
+            continue
+        act = d.get("GRBM_GUI_ACTIVE", 0.0) / 8.0
+        t = dur[k]
+        f64 = d.get("SQ_INSTS_VALU_MFMA_MOPS_F64", 0.0) * 512 / t / 1e12
+        b16 = d.get("SQ_INSTS_VALU_MFMA_MOPS_BF16", 0.0) * 512 / t / 1e12
+        ml.append(f"{k:52s} {int(d['dispatches']):5d} {t * 1e3:8.2f} {100 * busy / (act * SIMD_NUM):11.2f} "
+                  f"{f64:9.2f} {100 * f64 / 78.6:6.2f} {b16:9.2f} {100 * b16 / 2500:7.2f}")
+    with open(os.path.join(ROOT, "profiles", "gj_sweep_c4_mfma.txt"), "w") as f:
+        f.write("\n".join(ml) + "\n")
+    print("\n".join(lines + [""] + ml))
+
+
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else None)
+    if len(sys.argv) > 1 and sys.argv[1] == "c4":
+        main_c4(sys.argv[2] if len(sys.argv) > 2 else None)
+    else:
+        main(sys.argv[1] if len(sys.argv) > 1 else None)
