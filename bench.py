@@ -116,6 +116,15 @@ def _timed_ms(fn, steps, warmup):
     return e0.elapsed_time(e1) / steps
 
 
+def _tune(plan, A, B):
+    """hipla_plan_autotune on the bench inputs (kernel shapes per size class)."""
+    import torch
+    X = torch.empty_like(B)
+    codes = torch.empty(plan.ncells, dtype=torch.int32, device=B.device)
+    plan.autotune(A, B, X, codes)
+    torch.cuda.synchronize()
+
+
 def _emit(out):
     print(json.dumps(out), flush=True)
     return 0
@@ -443,6 +452,7 @@ def main():
         # reads and writes 512 MiB (the calibration for HBM byte counters); prints the roofline
         # object alone
         fp64 = pa.Plan.for_batch(batch, mode="fp64", graph=False)
+        _tune(fp64, A, B)
         roof, classes = measure_roofline(fp64, A, B)
         src = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
@@ -478,8 +488,9 @@ def main():
     ms_mixed, wall_mixed, Xm, cm, sm = timed(mixed, args.steps, args.warmup, stats=True)
     print(f"[bench] mixed: {ms_mixed:.3f} ms/step", file=sys.stderr, flush=True)
 
-    # ---- batched FP64 GSL order, graph replay ----
+    # ---- batched FP64 GSL order (kernel shapes tuned per size class), graph replay ----
     fp64 = pa.Plan.for_batch(batch, mode="fp64", graph=True)
+    _tune(fp64, A, B)
     ms_fp64, _, Xf, cf, _ = timed(fp64, args.steps, args.warmup)
     print(f"[bench] fp64 batched: {ms_fp64:.3f} ms/step", file=sys.stderr, flush=True)
 
@@ -571,7 +582,7 @@ def main():
         "speedup_vs_plain_fp64": None if ms_plain is None else round(ms_plain / ms_mixed, 2),
         "roofline": roof,
         "cpu_baseline": cpu,
-        "groups": [{k: g2[k] for k in ("kernel", "fp64_kernel", "nmax", "count", "direct_fp64")} |
+        "groups": [{k: g2[k] for k in ("kernel", "fp64_kernel", "nmax", "count", "direct_fp64", "variant")} |
                    {"ms": round(float(t), 4)} for g2, t in zip(groups, gms)],
         "fp64_classes": classes,
         "fallback_pass_ms": round(float(gms[len(groups)]), 4),

@@ -208,11 +208,22 @@ int hipla_ion_destroy (hipla_ion ctx);
 int hipla_ion_populations (hipla_ion ctx, const hipla_ion_rates *rates, double *dDensity, double *dNe, int *dStatus,
                            int *dIters, int *dCodes, void *stream);
 
-/* Time the kernel variants for this plan's size histogram and keep the fastest per size class
- * (the choice is cached per process and, if HIPLA_TUNE_CACHE names a file, across processes).
- * Runs synchronously on `stream`.  dA..dX must be valid inputs as for hipla_solve. */
+/* Time the kernel variants for this plan's size histogram and keep the fastest per size class:
+ * in mixed plans FP32 LU + FP64 refinement against FP64 straight away, and in both modes the FP64
+ * kernel shape (hipla_plan_set_variant) of every class solved in FP64.  The choices are cached per
+ * process, keyed on (kernel family, largest order, cell count), and, if HIPLA_TUNE_CACHE names a
+ * file, across processes.  Runs synchronously on `stream`.  dA..dX must be valid inputs as for
+ * hipla_solve. */
 int hipla_plan_autotune (hipla_plan plan, const double *dA, const double *dB, double *dX, int *dcodes,
                          void *stream);
+
+/* The FP64 kernel shape of size-class group `group` (hipla_plan_describe lists the groups and
+ * their "variant"), as hipla_plan_autotune would choose it: 0 the default; 1 the other shape of
+ * the group's family (two waves per workgroup instead of four -- one instead of two for n <= 64 --
+ * for the one-wave-per-matrix kernels, the blocked kernel instead of the register-tiled one for
+ * 64 < n <= 175, 512 threads instead of 256 for the blocked kernel at n <= 256).  Every shape gives
+ * the same values.  0, or 5 for a shape the group does not have. */
+int hipla_plan_set_variant (hipla_plan plan, int group, int variant);
 
 /* Profiling helper (no reference counterpart): launch each size-class group of the plan alone
  * `reps` times on the plan's own stream, bracketed by HIP events, and write the average ms per

@@ -66,6 +66,7 @@ SIGNATURES = {
                                       c_void_p, c_void_p]),
     "hipla_plan_autotune": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "hipla_plan_describe": (c_int, [c_void_p, ctypes.c_char_p, c_int]),
+    "hipla_plan_set_variant": (c_int, [c_void_p, c_int, c_int]),
     "hipla_plan_time_groups": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, _dp, c_int]),
 }
 

@@ -29,6 +29,12 @@ int wgr_f64_limit ();   // largest n of the workgroup-resident FP64 route
 int tile_f64_limit ();
 hipError_t launch_tile_f64 (const SolveArgs &a, int nmax, bool keep, int grid, hipStream_t st);
 void tile_kernel_symbol (int nmax, bool keep, char *buf, size_t len);
+// kernel shapes of an FP64 family for the autotuner (k_solve_f64.hip)
+int fp64_family_of (int nmax);
+int fp64_variants (int family, int nmax);
+hipError_t launch_fp64_variant (const SolveArgs &a, int nmax, int variant, bool keep, int grid, hipStream_t st,
+                                bool with_bsub = true);
+void fp64_variant_symbol (int nmax, int variant, bool keep, char *buf, size_t len);
 void fp64_kernel_symbol (int nmax, bool keep, char *buf, size_t len);
 
 // ---- FP64 MFMA Gauss-Jordan inverse of row-diagonally dominant cells (k_gj.hip) ----

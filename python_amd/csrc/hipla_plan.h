@@ -66,6 +66,7 @@ struct Group
   int grid = 0;       // persistent grid size (blocked kernels)
   int stream = 0;     // fork-join branch
   int direct = 0;     // mixed plans: 1 = the tuner routes this size class straight to FP64
+  int variant = 0;    // FP64 kernel shape chosen by the tuner (launch_fp64_variant)
   double t_mixed = -1.0, t_direct = -1.0;   // tuner measurements (ms), -1 if not tuned
 };
 

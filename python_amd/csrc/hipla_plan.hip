@@ -324,13 +324,14 @@ extern "C" int hipla_plan_describe (hipla_plan p, char *buf, int buflen)
     // the FP64 kernel instantiation that runs the group (FP64 plans, or a mixed size class
     // routed straight to FP64)
     char sym[96];
-    fp64_kernel_symbol (g.nmax, (p->flags & HIPLA_FLAG_KEEP_FACTORS) != 0, sym, sizeof sym);
+    fp64_variant_symbol (g.nmax, g.variant, (p->flags & HIPLA_FLAG_KEEP_FACTORS) != 0, sym, sizeof sym);
     char line2[512];
     snprintf (line2, sizeof line2,
               "%s{\"kernel\": \"%s\", \"param\": %d, \"nmax\": %d, \"count\": %d, \"grid\": %d, "
-              "\"direct_fp64\": %d, \"t_mixed_ms\": %.4f, \"t_direct_ms\": %.4f, \"bytes\": %.0f, "
+              "\"direct_fp64\": %d, \"variant\": %d, \"t_mixed_ms\": %.4f, \"t_direct_ms\": %.4f, \"bytes\": %.0f, "
               "\"flops\": %.0f, \"fp64_kernel\": \"%s\"}",
-              i ? ", " : "", kind_name (g.kind), g.param, g.nmax, g.count, g.grid, g.direct, g.t_mixed, g.t_direct,
+              i ? ", " : "", kind_name (g.kind), g.param, g.nmax, g.count, g.grid, g.direct, g.variant, g.t_mixed,
+              g.t_direct,
               bytes, flops, sym);
     s += line2;
   }
@@ -369,11 +370,10 @@ static hipError_t launch_group_f64 (hipla_plan p, const SolveArgs &a, const Grou
   switch (g.kind)
   {
   case KK_WAVE_F64:
-    return launch_wave_f64 (a, g.param, keep, 0, st);
   case KK_WGR_F64:
-    return launch_wgr_f64 (a, g.nmax, keep, 0, st);
+    return launch_fp64_variant (a, g.nmax, g.variant, keep, 0, st);
   case KK_BLOCKED_F64:
-    return launch_blocked_f64 (a, g.nmax, std::min (g.grid, std::max (a.nlist, 1)), keep, st);
+    return launch_fp64_variant (a, g.nmax, g.variant, keep, std::min (g.grid, std::max (a.nlist, 1)), st);
   default:
     return hipErrorInvalidValue;
   }
@@ -406,7 +406,7 @@ int plan_enqueue_solve (hipla_plan p, const double *dA, const double *dB, double
     const Group &g = p->groups.back ();
     SolveArgs a = make_args (p, dA, dB, dX, dcodes, g);
     const bool keep = (p->flags & HIPLA_FLAG_KEEP_FACTORS) != 0;
-    HIPLA_CHECK (launch_blocked_f64 (a, g.nmax, std::min (g.grid, std::max (a.nlist, 1)), keep, st, false));
+    HIPLA_CHECK (launch_fp64_variant (a, g.nmax, g.variant, keep, std::min (g.grid, std::max (a.nlist, 1)), st, false));
     HIPLA_CHECK (hipEventRecord (p->ev_fork, st));
     HIPLA_CHECK (hipStreamWaitEvent (p->fork_st, p->ev_fork, 0));
     HIPLA_CHECK (launch_bsub_cells (a, g.nmax, p->fork_st));

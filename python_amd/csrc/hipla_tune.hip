@@ -140,6 +140,12 @@ int time_fallback (hipla_plan p, const double *dA, const double *dB, double *dX,
 }
 } // namespace
 
+namespace
+{
+int tune_variant (hipla_plan p, int gi, const double *dA, const double *dB, double *dX, int *dcodes, int reps,
+                  hipStream_t st);
+}
+
 extern "C" int hipla_plan_time_groups (hipla_plan p, const double *dA, const double *dB, double *dX, int *dcodes,
                                        hipla_cell_stats *dstats, int reps, double *ms_out, int nout)
 {
@@ -183,6 +189,47 @@ extern "C" int hipla_plan_time_groups (hipla_plan p, const double *dA, const dou
   return 0;
 }
 
+namespace
+{
+// The FP64 kernel shape of one group: every variant of its family (launch_fp64_variant: waves
+// per workgroup of the one-wave-per-matrix kernels, register tiles or blocked for the middle
+// orders, 256 or 512 threads of the blocked kernel) timed alone, the fastest kept.  All variants
+// give the same values (GSL's operation order).
+int tune_variant (hipla_plan p, int gi, const double *dA, const double *dB, double *dX, int *dcodes, int reps,
+                  hipStream_t st)
+{
+  Group &g = p->groups[gi];
+  const int fam = fp64_family_of (g.nmax);
+  const int nv = fp64_variants (fam, g.nmax);
+  if (nv <= 1)
+    return 0;
+  const Key key (300 + fam, (p->flags & HIPLA_FLAG_KEEP_FACTORS) ? 1 : 0, g.nmax, g.count);
+  auto it = tune_cache.find (key);
+  if (it != tune_cache.end ())
+  {
+    g.variant = it->second;
+    return 0;
+  }
+  int best = 0, rc;
+  double tbest = 1e300;
+  for (int v = 0; v < nv; v++)
+  {
+    g.variant = v;
+    double t = 0.0;
+    if ((rc = time_group (p, gi, dA, dB, dX, dcodes, nullptr, reps, st, t)))
+      return rc;
+    if (t < tbest)
+    {
+      tbest = t;
+      best = v;
+    }
+  }
+  g.variant = best;
+  store_locked (key, best);
+  return 0;
+}
+} // namespace
+
 extern "C" int hipla_plan_autotune (hipla_plan p, const double *dA, const double *dB, double *dX, int *dcodes,
                                     void *stream)
 {
@@ -191,14 +238,22 @@ extern "C" int hipla_plan_autotune (hipla_plan p, const double *dA, const double
     hipla_set_detail ("hipla_plan_autotune: bad argument");
     return 5;
   }
-  if (p->mode != HIPLA_MODE_MIXED)
-    return 0;   // FP64 plans have one shape per size class
+  if (p->flags & HIPLA_FLAG_PER_CELL)
+    return 0;   // the one-launch-per-matrix baseline keeps its shapes
   hipStream_t st = stream ? (hipStream_t) stream : p->aux.back ();
   HIPLA_CHECK (hipStreamSynchronize (st));
   std::lock_guard<std::mutex> lk (tune_mu);
   load_file_locked ();
   const int reps = 3;
   int rc;
+  if (p->mode != HIPLA_MODE_MIXED)
+  {
+    for (int gi = 0; gi < (int) p->groups.size (); gi++)
+      if ((rc = tune_variant (p, gi, dA, dB, dX, dcodes, reps, st)))
+        return rc;
+    drop_graph (p);
+    return 0;
+  }
   for (int gi = 0; gi < (int) p->groups.size (); gi++)
   {
     Group &g = p->groups[gi];
@@ -225,6 +280,32 @@ extern "C" int hipla_plan_autotune (hipla_plan p, const double *dA, const double
     g.t_direct = t_dir;
     store_locked (key, g.direct);
   }
+  // the FP64 kernel shape of every size class that goes straight to FP64
+  for (int gi = 0; gi < (int) p->groups.size (); gi++)
+  {
+    const Group &g = p->groups[gi];
+    if ((g.kind == KK_BLOCKED_MIX || g.direct) && (rc = tune_variant (p, gi, dA, dB, dX, dcodes, reps, st)))
+      return rc;
+  }
+  drop_graph (p);
+  return 0;
+}
+
+extern "C" int hipla_plan_set_variant (hipla_plan p, int group, int variant)
+{
+  if (!p || group < 0 || group >= (int) p->groups.size () || variant < 0)
+  {
+    hipla_set_detail ("hipla_plan_set_variant: bad argument");
+    return 5;
+  }
+  Group &g = p->groups[group];
+  if (variant >= fp64_variants (fp64_family_of (g.nmax), g.nmax))
+  {
+    hipla_set_detail ("hipla_plan_set_variant: group %d (n <= %d) has %d kernel shapes", group, g.nmax,
+                      fp64_variants (fp64_family_of (g.nmax), g.nmax));
+    return 5;
+  }
+  g.variant = variant;
   drop_graph (p);
   return 0;
 }

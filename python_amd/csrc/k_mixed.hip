@@ -932,12 +932,8 @@ static SolveArgs group_args (hipla_plan p, const double *dA, const double *dB, d
 static int launch_group_direct_fp64 (hipla_plan p, const SolveArgs &a, const Group &g, hipla_cell_stats *dstats,
                                      hipStream_t s)
 {
-  if (g.nmax <= 64)
-    HIPLA_CHECK (launch_wave_f64 (a, g.nmax <= 16 ? 16 : g.nmax <= 32 ? 32 : 64, false, 0, s));
-  else if (g.nmax <= wgr_f64_limit ())
-    HIPLA_CHECK (launch_wgr_f64 (a, g.nmax, false, 0, s));
-  else
-    HIPLA_CHECK (launch_blocked_f64 (a, g.nmax, std::max (1, std::min (g.count, g.grid)), false, s));
+  HIPLA_CHECK (launch_fp64_variant (a, g.nmax, g.variant, false,
+                                    g.nmax > wgr_f64_limit () ? std::max (1, std::min (g.count, g.grid)) : 0, s));
   if (dstats)
   {
     hipLaunchKernelGGL (k_mark_fp64_stats, dim3 ((g.count + 255) / 256), dim3 (256), 0, s, a.list, g.count, dstats);
@@ -1045,7 +1041,7 @@ int plan_enqueue_mixed (hipla_plan p, const double *dA, const double *dB, double
       for (; gj < nb; gj++)
       {
         const Group &h = p->groups[gj];
-        if (fp64_family (h.nmax) != fp64_family (u.nmax) || h.first != u.first + u.count)
+        if (fp64_family (h.nmax) != fp64_family (u.nmax) || h.variant != u.variant || h.first != u.first + u.count)
           break;
         u.count += h.count;
         u.nmax = std::max (u.nmax, h.nmax);
@@ -1059,7 +1055,7 @@ int plan_enqueue_mixed (hipla_plan p, const double *dA, const double *dB, double
     {
       const Group &u = units.back ();
       SolveArgs a = group_args (p, dA, dB, dX, dcodes, u);
-      HIPLA_CHECK (launch_blocked_f64 (a, u.nmax, std::max (1, u.grid), false, st, false));
+      HIPLA_CHECK (launch_fp64_variant (a, u.nmax, u.variant, false, std::max (1, u.grid), st, false));
       if (dstats)
       {
         hipLaunchKernelGGL (k_mark_fp64_stats, dim3 ((u.count + 255) / 256), dim3 (256), 0, st, a.list, u.count,
@@ -1098,7 +1094,7 @@ int plan_enqueue_mixed (hipla_plan p, const double *dA, const double *dB, double
     {
       const Group &h = p->groups[gj];
       if (!(h.kind == KK_BLOCKED_MIX || h.direct) || fp64_family (h.nmax) != fp64_family (u.nmax) ||
-          h.first != u.first + u.count)
+          h.variant != u.variant || h.first != u.first + u.count)
         break;
       u.count += h.count;
       u.nmax = std::max (u.nmax, h.nmax);

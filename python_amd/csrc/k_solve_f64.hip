@@ -1226,3 +1226,76 @@ hipError_t setup_solve_f64_attrs (size_t max_smem)
   set_smem_attr (k_bsub_f64<BLK_MAXM>, max_smem);
   return hipSuccess;
 }
+
+// Kernel shapes of one FP64 kernel family (equal values: every variant keeps GSL's order), for
+// the autotuner.  Family f = 0, 1, 2 (n <= 16, 32, 64): one wave per matrix with WPB waves per
+// workgroup, variant 0 the default WPB, 1 the other; family 3 (64 < n <= wgr_f64_limit ()):
+// variant 0 the register-tiled kernel, 1 the blocked kernel; family 4: the blocked kernel with
+// 256 (variant 0) or 512 (variant 1, n <= 256 only) threads.
+int fp64_family_of (int nmax)
+{
+  return nmax <= 16 ? 0 : nmax <= 32 ? 1 : nmax <= 64 ? 2 : nmax <= wgr_f64_limit () ? 3 : 4;
+}
+
+int fp64_variants (int family, int nmax)
+{
+  return family < 4 ? 2 : (nmax <= 256 ? 2 : 1);
+}
+
+hipError_t launch_fp64_variant (const SolveArgs &a, int nmax, int variant, bool keep, int grid, hipStream_t st,
+                                bool with_bsub)
+{
+  if (a.nlist <= 0)
+    return hipSuccess;
+  const int fam = fp64_family_of (nmax);
+  if (variant <= 0 || variant >= fp64_variants (fam, nmax))
+  {
+    if (fam < 3)
+      return launch_wave_f64 (a, fam == 0 ? 16 : fam == 1 ? 32 : 64, keep, grid, st);
+    if (fam == 3)
+    {
+      hipError_t e = launch_tile_f64 (a, nmax, keep, grid, st);
+      return (e != hipSuccess || !with_bsub) ? e : launch_bsub_f64 (a, nmax, st);
+    }
+    return launch_blocked_f64 (a, nmax, grid > 0 ? grid : std::min (a.nlist, blocked_f64_wgs_per_cu (nmax) * hipla_device_info ().num_cu), keep, st, with_bsub);
+  }
+  switch (fam)
+  {
+  case 0:
+    return launch_wave_nc<16, 2> (a, keep, grid, st);
+  case 1:
+    return launch_wave_nc<32, 2> (a, keep, grid, st);
+  case 2:
+    return launch_wave_nc<64, 1> (a, keep, grid, st);
+  case 3:
+    return launch_blocked_f64 (a, nmax, grid > 0 ? grid : std::min (a.nlist, blocked_f64_wgs_per_cu (nmax) * hipla_device_info ().num_cu), keep, st, with_bsub);
+  default:
+  {
+    if (!a.F || !a.Y)
+      return hipErrorInvalidValue;
+    const int g2 = grid > 0 ? grid : std::min (a.nlist, blocked_f64_wgs_per_cu (nmax) * hipla_device_info ().num_cu);
+    launch_blocked_m<BLK_BS, 16, true> (a, g2, blocked_f64_smem (nmax), keep, st);
+    hipError_t e = hipGetLastError ();
+    return (e != hipSuccess || !with_bsub) ? e : launch_bsub_f64 (a, nmax, st);
+  }
+  }
+}
+
+// Name of the kernel instantiation(s) a variant runs (as profilers print it)
+void fp64_variant_symbol (int nmax, int variant, bool keep, char *buf, size_t len)
+{
+  const int fam = fp64_family_of (nmax);
+  if (variant <= 0 || variant >= fp64_variants (fam, nmax))
+  {
+    fp64_kernel_symbol (nmax, keep, buf, len);
+    return;
+  }
+  const char *k = keep ? "true" : "false";
+  const int m = nmax <= 128 ? 2 : nmax <= 192 ? 3 : nmax <= 256 ? 4 : 8;
+  if (fam < 3)
+    snprintf (buf, len, "k_wave_f64<%d, %d, %s>", fam == 0 ? 16 : fam == 1 ? 32 : 64, fam == 2 ? 1 : 2, k);
+  else if (fam == 3)
+    snprintf (buf, len, "k_blocked_f64<256, 16, true, %s> + k_bsub_f64<%d>", k, m);
+  else
+    snprintf (buf, len, "k_blocked_f64<%d, 16, true, %s> + k_bsub_f64<%d>", BLK_BS, k, m);
+}

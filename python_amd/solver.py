@@ -117,6 +117,10 @@ class Plan:
     def for_batch(cls, batch, **kw):
         return cls(batch.n, batch.offA, batch.offV, **kw)
 
+    def set_variant(self, group: int, variant: int):
+        """FP64 kernel shape of one size-class group (hipla_plan_set_variant)."""
+        check(lib().hipla_plan_set_variant(self._h, int(group), int(variant)), "hipla_plan_set_variant")
+
     def close(self):
         if getattr(self, "_h", None):
             lib().hipla_plan_destroy(self._h)
