@@ -1029,10 +1029,11 @@ int plan_enqueue_mixed (hipla_plan p, const double *dA, const double *dB, double
   const DeviceInfo &dev = hipla_device_info ();
   if (!any_mixed)
   {
-    // every group straight to FP64: merge, then run the last unit first when it is a blocked
-    // one.  Its factor kernel fills the chip; its back substitution (one latency-bound wave per
-    // cell, a fraction of each SIMD) then runs on the fork stream beside the smaller classes,
-    // and the caller's stream waits for it at the end.
+    // every group straight to FP64: merge.  When the last unit is a blocked one, it and its back
+    // substitution run on the fork stream from the start while the smaller units run on the
+    // caller's stream: the smaller kernels take the CU slots the blocked kernel's persistent
+    // workgroups leave as they finish, and the back substitution (one latency-bound wave per
+    // cell) overlaps them too.  The caller's stream waits for the fork at the end.
     std::vector<Group> units;
     for (int gi = 0; gi < nb;)
     {
@@ -1055,16 +1056,15 @@ int plan_enqueue_mixed (hipla_plan p, const double *dA, const double *dB, double
     {
       const Group &u = units.back ();
       SolveArgs a = group_args (p, dA, dB, dX, dcodes, u);
-      HIPLA_CHECK (launch_fp64_variant (a, u.nmax, u.variant, false, std::max (1, u.grid), st, false));
-      if (dstats)
-      {
-        hipLaunchKernelGGL (k_mark_fp64_stats, dim3 ((u.count + 255) / 256), dim3 (256), 0, st, a.list, u.count,
-                            dstats);
-        HIPLA_CHECK (hipGetLastError ());
-      }
       HIPLA_CHECK (hipEventRecord (p->ev_fork, st));
       HIPLA_CHECK (hipStreamWaitEvent (p->fork_st, p->ev_fork, 0));
-      HIPLA_CHECK (launch_bsub_cells (a, u.nmax, p->fork_st));
+      HIPLA_CHECK (launch_fp64_variant (a, u.nmax, u.variant, false, std::max (1, u.grid), p->fork_st, true));
+      if (dstats)
+      {
+        hipLaunchKernelGGL (k_mark_fp64_stats, dim3 ((u.count + 255) / 256), dim3 (256), 0, p->fork_st, a.list,
+                            u.count, dstats);
+        HIPLA_CHECK (hipGetLastError ());
+      }
       HIPLA_CHECK (hipEventRecord (p->ev_join, p->fork_st));
       units.pop_back ();
     }
