@@ -167,11 +167,14 @@ extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, con
   }
   std::vector<int> idx (ncells);
   std::iota (idx.begin (), idx.end (), 0);
+  // Groups of one kind in ascending param, except the mixed plan's column-block groups: those are
+  // in descending param, so that when adjacent groups are merged into one FP64 launch its list
+  // still runs largest cells first.
   std::stable_sort (idx.begin (), idx.end (), [&] (int a, int b) {
     if (kind[a] != kind[b])
       return kind[a] < kind[b];
     if (param[a] != param[b])
-      return param[a] < param[b];
+      return kind[a] == KK_LDS_MIX ? param[a] > param[b] : param[a] < param[b];
     return n[a] > n[b];
   });
   p->list = idx;

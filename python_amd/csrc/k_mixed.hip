@@ -25,6 +25,7 @@
 #include "hipla_plan.h"
 #include "hipla_tune.h"
 
+#include <algorithm>
 #include <utility>
 
 struct MixedArgs
@@ -1051,9 +1052,16 @@ int plan_enqueue_mixed (hipla_plan p, const double *dA, const double *dB, double
       units.push_back (u);
       gi = gj;
     }
-    const bool fork = units.size () > 1 && units.back ().nmax > wgr_f64_limit ();
+    // the unit with the largest cells, when it runs the blocked kernel
+    size_t big = 0;
+    for (size_t i = 1; i < units.size (); i++)
+      if (units[i].nmax > units[big].nmax)
+        big = i;
+    const bool fork = units.size () > 1 && units[big].nmax > wgr_f64_limit ();
     if (fork)
     {
+      // to the back, the others keeping their order
+      std::rotate (units.begin () + big, units.begin () + big + 1, units.end ());
       const Group &u = units.back ();
       SolveArgs a = group_args (p, dA, dB, dX, dcodes, u);
       HIPLA_CHECK (hipEventRecord (p->ev_fork, st));
