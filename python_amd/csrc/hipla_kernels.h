@@ -44,7 +44,9 @@ struct GjArgs
   const int *list;      // the cells of one size class
   int ncells;           // in list
   int nsz;              // their order (a multiple of 64, >= 128)
-  int sb;               // diagonal block of the step
+  int sb;               // first diagonal block of the step
+  int sw;               // diagonal blocks of the step (1 or 2)
+  int r0, nr;           // the step's block range [r0, r0 + nr) (a sub-sweep of a 2 x 2 block: nr = 2)
   const double *A;      // input
   double *M;            // output, inverted in place
   int *codes;

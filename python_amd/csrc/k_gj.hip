@@ -10,9 +10,11 @@
 //   2. M[S,O] <- D M[S,O]              k_gj_tile<2>
 //   3. M[O,O] <- M[O,O] - M[O,S] M[S,O] k_gj_tile<4>  (M[O,S] still the old values)
 //   4. M[O,S] <- -M[O,S] D             k_gj_tile<5>
-// After the last step M holds M^-1.  Every product is a 64 x 64 x 64 block product on
-// v_mfma_f64_16x16x4f64 (four waves, a 32 x 32 quarter each, operands staged through LDS in two
-// K halves; lane l holds A[l % 16][l / 16] and B[l / 16][l % 16] of a 16 x 4 x 16 step and
+// After the last step M holds M^-1.  Steps normally pivot on two blocks at once (S of 128
+// columns; its diagonal block inverted first by the same sweep over its own 2 x 2 blocks), which
+// halves the HBM traffic of step 3 per flop.  Every product is a 64 x 64 output block with K = 64
+// or 128 on v_mfma_f64_16x16x4f64 (four waves, a 32 x 32 quarter each, operands staged through LDS
+// in slabs of 32 k; lane l holds A[l % 16][l / 16] and B[l / 16][l % 16] of a 16 x 4 x 16 step and
 // element e of its result at row l / 16 + 4 e, column l % 16).  The sweep does not pivot: it runs on cells whose A is row diagonally dominant
 // (|a_ii| >= sum_j!=i |a_ij|, which N = I - Q with substochastic Q is).  M is then column
 // diagonally dominant, every Schur complement stays so, and partial pivoting would never swap a
@@ -28,9 +30,15 @@
 #include <cstdlib>
 
 #define GJ_NB 64                  // block order
-#define GJ_KC 32                  // K rows staged in LDS per half
-#define GJ_PA (GJ_NB + 4)         // LDS pitch of the A half (k-major: [k][row])
-#define GJ_PB (GJ_KC + 2)         // LDS pitch of the B half (column-major: [col][k])
+#ifndef GJ_KC
+#define GJ_KC 32                  // K rows staged in LDS per slab
+#endif
+#ifndef GJ_OCC
+#define GJ_OCC 4                  // tile workgroups per CU (launch bounds)
+#endif
+#define GJ_NE (GJ_KC / 4)         // elements of each operand slab per thread
+#define GJ_PA (GJ_NB + 4)         // LDS pitch of the A slab (k-major: [k][row])
+#define GJ_PB (GJ_KC + 2)         // LDS pitch of the B slab (column-major: [col][k])
 
 typedef double gj_d4 __attribute__ ((ext_vector_type (4)));
 
@@ -154,131 +162,180 @@ __device__ __forceinline__ int xcd_item (int bid, int G)
   return x < rem ? x * (q0 + 1) + i : rem * (q0 + 1) + (x - rem) * q0 + i;
 }
 
-// ---- steps 2-4: one 64 x 64 block product per workgroup ----
-//   MODE 2: C(s, tj) = D C(s, tj)           MODE 4: C(ti, tj) -= M(ti, s) M(s, tj)
-//   MODE 5: C(ti, s) = -C(ti, s) D
-// The two K halves of the operands pass through registers: the second half's global loads are
-// in flight while the first runs on the matrix cores.  Four workgroups share a CU (35 KB of LDS
-// and under 128 VGPRs each), so one's loads overlap the others' products.
-template <int MODE>
-__global__ __launch_bounds__ (256, 4) void k_gj_tile (GjArgs g)
+// ---- steps 2-4: block products over the sweep's block range ----
+// A step pivots on the W = g.sw diagonal blocks S = sb .. sb+W-1 (one or two; with two, M[S,S]
+// has already been inverted in place by a two-step sweep of its 2 x 2 blocks), over the blocks
+// O = [r0, r0 + nr) minus S.  Block rows and columns of O are enumerated by
+// o -> r0 + o (+ W past S).  One workgroup forms RB x CB output blocks with K = 64 W:
+//   MODE 2: C(S, tj) = D C(S, tj)                (RB = W, CB = 1; one workgroup per tj in O)
+//   MODE 4: C(ti, tj) -= M(ti, S) M(S, tj)       (RB = CB = 1; one per (ti, tj) in O x O)
+//   MODE 5: C(ti, S) = -C(ti, S) D               (RB = 1, CB = W; one per ti in O)
+// MODES 2 and 5 read the blocks they overwrite, so a workgroup computes all W of them before it
+// writes any (W = g.sw is a template parameter).  Operands pass through LDS in slabs of GJ_KC k-rows, the next slab's global loads
+// in flight while the current one runs on the matrix cores; each wave forms a 32 x 32 quarter of
+// every output block.
+__device__ __forceinline__ int gj_oblock (int o, int r0, int sb, int w)
 {
-  const int nb = g.nsz / GJ_NB;
-  const int tpc = MODE == 4 ? (nb - 1) * (nb - 1) : nb - 1;
+  const int b = r0 + o;
+  return b >= sb ? b + w : b;
+}
+
+template <int MODE, int W>
+__global__ __launch_bounds__ (256, (MODE == 4 || W == 1) ? GJ_OCC : 2) void k_gj_tile (GjArgs g)
+{
+  constexpr int RB = MODE == 2 ? W : 1, CB = MODE == 5 ? W : 1;
+  constexpr int NRA = GJ_KC * GJ_NB * RB / 256;   // A elements per thread and slab
+  constexpr int NRB = GJ_KC * GJ_NB * CB / 256;   // B elements per thread and slab
+  constexpr int PA = GJ_NB * RB + 4;              // LDS pitch of the A slab ([k][row])
+  constexpr int w = W;                            // g.sw
+  const int no = g.nr - w;                        // blocks of O
+  const int tpc = MODE == 4 ? no * no : no;
   const int item = xcd_item (blockIdx.x, gridDim.x);
   const int ci = item / tpc, t = item - ci * tpc;
   const int cell = g.list[ci];
   if (g.flag[cell])
     return;
   const int n = g.nsz, sb = g.sb;
-  int ti, tj;
+  int ai, bj;                                     // first block row of A, first block column of B
   if (MODE == 4)
   {
-    ti = t / (nb - 1);
-    tj = t - ti * (nb - 1);
-    ti += ti >= sb;
-    tj += tj >= sb;
+    const int oi = t / no;
+    ai = gj_oblock (oi, g.r0, sb, w);
+    bj = gj_oblock (t - oi * no, g.r0, sb, w);
   }
   else if (MODE == 2)
   {
-    ti = sb;
-    tj = t + (t >= sb);
+    ai = sb;
+    bj = gj_oblock (t, g.r0, sb, w);
   }
   else
   {
-    ti = t + (t >= sb);
-    tj = sb;
+    ai = gj_oblock (t, g.r0, sb, w);
+    bj = sb;
   }
-  const int ai = MODE == 2 ? sb : ti;     // block row of the left operand (its block column is sb)
-  const int bj = MODE == 5 ? sb : tj;     // block column of the right operand (block row sb)
+  constexpr int K = GJ_NB * W;
   const double sgn = MODE == 2 ? 1.0 : -1.0;
   double *m = g.M + g.offA[cell];
-  __shared__ double As[GJ_KC][GJ_PA];     // [k][row] of the left operand, sign applied
-  __shared__ double Bs[GJ_NB][GJ_PB];     // [col][k] of the right operand
+  __shared__ double As[GJ_KC][PA];                // [k][row] of the left operand, sign applied
+  __shared__ double Bs[GJ_NB * CB][GJ_PB];        // [col][k] of the right operand
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int rw = (wv >> 1) * 32, cw = (wv & 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
 
-  // this thread's eight elements of each operand half: A (k = idx >> 6, row = idx & 63) and
-  // B (col = idx >> 5, k = idx & 31) for idx = tid + 256 e
+  // A(k, row) = abase[k n + row] (rows of the RB blocks from ai on), B(col, k) = bbase[col n + k]
+  // (columns of the CB blocks from bj on); this thread's slab elements are idx = tid + 256 e:
+  // A (k = idx / (64 RB), row = idx % (64 RB)), B (col = idx / GJ_KC, k = idx % GJ_KC)
   const double *abase = m + (size_t) (sb * GJ_NB) * n + ai * GJ_NB;
   const double *bbase = m + (size_t) (bj * GJ_NB) * n + sb * GJ_NB;
-  double ra[8], rb[8];
+  double ra[NRA], rb[NRB];
 #pragma unroll
-  for (int e = 0; e < 8; e++)
+  for (int e = 0; e < NRA; e++)
   {
-    const int ia = tid + 256 * e, ib = tid + 256 * e;
-    ra[e] = abase[(size_t) (ia >> 6) * n + (ia & 63)];
-    rb[e] = bbase[(size_t) (ib >> 5) * n + (ib & 31)];
+    const int idx = tid + 256 * e;
+    ra[e] = abase[(size_t) (idx / (GJ_NB * RB)) * n + (idx % (GJ_NB * RB))];
+  }
+#pragma unroll
+  for (int e = 0; e < NRB; e++)
+  {
+    const int idx = tid + 256 * e;
+    rb[e] = bbase[(size_t) (idx / GJ_KC) * n + (idx % GJ_KC)];
   }
 
-  gj_d4 acc[2][2];
+  gj_d4 acc[RB][CB][2][2];
 #pragma unroll
-  for (int bi = 0; bi < 2; bi++)
+  for (int pr = 0; pr < RB; pr++)
 #pragma unroll
-    for (int bq = 0; bq < 2; bq++)
-    {
-      if (MODE == 4)
-      {
-        // v_mfma_f64_16x16x4f64 keeps element e of lane l at row l/16 + 4 e, column l % 16
-        const double *src = m + (size_t) (tj * GJ_NB + cw + 16 * bq + lr) * n + ti * GJ_NB + rw + 16 * bi + lk;
-        acc[bi][bq] = gj_d4{ src[0], src[4], src[8], src[12] };
-      }
-      else
-        acc[bi][bq] = gj_d4{ 0.0, 0.0, 0.0, 0.0 };
-    }
+    for (int pc = 0; pc < CB; pc++)
+#pragma unroll
+      for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int bq = 0; bq < 2; bq++)
+        {
+          if (MODE == 4)
+          {
+            // v_mfma_f64_16x16x4f64 keeps element e of lane l at row l/16 + 4 e, column l % 16
+            const double *src =
+                m + (size_t) (bj * GJ_NB + cw + 16 * bq + lr) * n + ai * GJ_NB + rw + 16 * bi + lk;
+            acc[pr][pc][bi][bq] = gj_d4{ src[0], src[4], src[8], src[12] };
+          }
+          else
+            acc[pr][pc][bi][bq] = gj_d4{ 0.0, 0.0, 0.0, 0.0 };
+        }
 
 #pragma unroll
-  for (int kh = 0; kh < GJ_NB; kh += GJ_KC)
+  for (int kh = 0; kh < K; kh += GJ_KC)
   {
 #pragma unroll
-    for (int e = 0; e < 8; e++)
+    for (int e = 0; e < NRA; e++)
     {
-      const int ia = tid + 256 * e, ib = tid + 256 * e;
-      As[ia >> 6][ia & 63] = sgn * ra[e];
-      Bs[ib >> 5][ib & 31] = rb[e];
+      const int idx = tid + 256 * e;
+      As[idx / (GJ_NB * RB)][idx % (GJ_NB * RB)] = sgn * ra[e];
+    }
+#pragma unroll
+    for (int e = 0; e < NRB; e++)
+    {
+      const int idx = tid + 256 * e;
+      Bs[idx / GJ_KC][idx % GJ_KC] = rb[e];
     }
     __syncthreads ();
-    if (kh == 0)
+    if (kh + GJ_KC < K)
     {
-      // the second half's loads, in flight during the first half's products
+      // the next slab's loads, in flight during this slab's products
 #pragma unroll
-      for (int e = 0; e < 8; e++)
+      for (int e = 0; e < NRA; e++)
       {
-        const int ia = tid + 256 * e, ib = tid + 256 * e;
-        ra[e] = abase[(size_t) (GJ_KC + (ia >> 6)) * n + (ia & 63)];
-        rb[e] = bbase[(size_t) (ib >> 5) * n + GJ_KC + (ib & 31)];
+        const int idx = tid + 256 * e;
+        ra[e] = abase[(size_t) (kh + GJ_KC + idx / (GJ_NB * RB)) * n + (idx % (GJ_NB * RB))];
+      }
+#pragma unroll
+      for (int e = 0; e < NRB; e++)
+      {
+        const int idx = tid + 256 * e;
+        rb[e] = bbase[(size_t) (idx / GJ_KC) * n + kh + GJ_KC + (idx % GJ_KC)];
       }
     }
 #pragma unroll
     for (int k = 0; k < GJ_KC; k += 4)
     {
-      double av[2], bv[2];
+      double av[RB][2], bv[CB][2];
 #pragma unroll
-      for (int bi = 0; bi < 2; bi++)
-        av[bi] = As[k + lk][rw + 16 * bi + lr];
+      for (int pr = 0; pr < RB; pr++)
 #pragma unroll
-      for (int bq = 0; bq < 2; bq++)
-        bv[bq] = Bs[cw + 16 * bq + lr][k + lk];
+        for (int bi = 0; bi < 2; bi++)
+          av[pr][bi] = As[k + lk][GJ_NB * pr + rw + 16 * bi + lr];
+#pragma unroll
+      for (int pc = 0; pc < CB; pc++)
+#pragma unroll
+        for (int bq = 0; bq < 2; bq++)
+          bv[pc][bq] = Bs[GJ_NB * pc + cw + 16 * bq + lr][k + lk];
+#pragma unroll
+      for (int pr = 0; pr < RB; pr++)
+#pragma unroll
+        for (int pc = 0; pc < CB; pc++)
+#pragma unroll
+          for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+            for (int bq = 0; bq < 2; bq++)
+              acc[pr][pc][bi][bq] =
+                  __builtin_amdgcn_mfma_f64_16x16x4f64 (av[pr][bi], bv[pc][bq], acc[pr][pc][bi][bq], 0, 0, 0);
+    }
+    __syncthreads ();
+  }
+  // every read of the output blocks (MODES 2 and 5 read the blocks they overwrite) is done
+#pragma unroll
+  for (int pr = 0; pr < RB; pr++)
+#pragma unroll
+    for (int pc = 0; pc < CB; pc++)
 #pragma unroll
       for (int bi = 0; bi < 2; bi++)
 #pragma unroll
         for (int bq = 0; bq < 2; bq++)
-          acc[bi][bq] = __builtin_amdgcn_mfma_f64_16x16x4f64 (av[bi], bv[bq], acc[bi][bq], 0, 0, 0);
-    }
-    __syncthreads ();
-  }
-  // every read of this tile (MODES 2 and 5 read the tile they overwrite) is done: write it
+        {
+          double *dst = m + (size_t) ((bj + pc) * GJ_NB + cw + 16 * bq + lr) * n + (ai + pr) * GJ_NB + rw + 16 * bi + lk;
 #pragma unroll
-  for (int bi = 0; bi < 2; bi++)
-#pragma unroll
-    for (int bq = 0; bq < 2; bq++)
-    {
-      double *dst = m + (size_t) (tj * GJ_NB + cw + 16 * bq + lr) * n + ti * GJ_NB + rw + 16 * bi + lk;
-#pragma unroll
-      for (int e = 0; e < 4; e++)
-        dst[4 * e] = acc[bi][bq][e];
-    }
+          for (int e = 0; e < 4; e++)
+            dst[4 * e] = acc[pr][pc][bi][bq][e];
+        }
 }
 
 __global__ void k_gj_reset (int *fb)
@@ -303,23 +360,71 @@ hipError_t gj_reset (int *fb, hipStream_t st)
   return hipGetLastError ();
 }
 
-// every launch of one size class: prepare, then the nb steps of the sweep
+// The products of one sweep step over the blocks [r0, r0 + nr), pivoting on blocks sb .. sb+w-1
+// whose diagonal block is already inverted in place.
+static void gj_products (GjArgs g, int sb, int w, int r0, int nr, hipStream_t st)
+{
+  const int no = nr - w;
+  if (no <= 0)
+    return;
+  g.sb = sb;
+  g.sw = w;
+  g.r0 = r0;
+  g.nr = nr;
+  if (w == 1)
+  {
+    hipLaunchKernelGGL ((k_gj_tile<2, 1>), dim3 (g.ncells * no), dim3 (256), 0, st, g);
+    hipLaunchKernelGGL ((k_gj_tile<4, 1>), dim3 (g.ncells * no * no), dim3 (256), 0, st, g);
+    hipLaunchKernelGGL ((k_gj_tile<5, 1>), dim3 (g.ncells * no), dim3 (256), 0, st, g);
+  }
+  else
+  {
+    hipLaunchKernelGGL ((k_gj_tile<2, 2>), dim3 (g.ncells * no), dim3 (256), 0, st, g);
+    hipLaunchKernelGGL ((k_gj_tile<4, 2>), dim3 (g.ncells * no * no), dim3 (256), 0, st, g);
+    hipLaunchKernelGGL ((k_gj_tile<5, 2>), dim3 (g.ncells * no), dim3 (256), 0, st, g);
+  }
+}
+
+// Steps of 128 columns (two blocks) halve the trailing update's HBM traffic per flop: each reads
+// and writes every other block once per 128 columns instead of once per 64.
+// HIPLA_GJ_STEP=64 in the environment keeps 64-column steps.
+static bool gj_wide ()
+{
+  static const bool on = [] {
+    const char *e = getenv ("HIPLA_GJ_STEP");
+    return !(e && std::atoi (e) == 64);
+  }();
+  return on;
+}
+
+// every launch of one size class: prepare, then the sweep.  A two-block step first inverts its
+// 128 x 128 diagonal block in place by a sweep of that block's own 2 x 2 blocks, then forms the
+// products of the whole step with K = 128.
 hipError_t gj_enqueue_class (const GjArgs &g0, hipStream_t st)
 {
   if (g0.ncells <= 0)
     return hipSuccess;
   GjArgs g = g0;
   const int nb = g.nsz / GJ_NB;
+  g.sb = 0;
+  g.sw = 1;
+  g.r0 = 0;
+  g.nr = nb;
   hipLaunchKernelGGL (k_gj_prepare, dim3 (g.ncells), dim3 (256), 0, st, g);
   hipError_t e = hipGetLastError ();
-  for (int s = 0; s < nb && e == hipSuccess; s++)
+  for (int s = 0; s < nb && e == hipSuccess;)
   {
-    g.sb = s;
-    hipLaunchKernelGGL (k_gj_diag, dim3 (g.ncells), dim3 (256), 0, st, g);
-    hipLaunchKernelGGL (k_gj_tile<2>, dim3 (g.ncells * (nb - 1)), dim3 (256), 0, st, g);
-    hipLaunchKernelGGL (k_gj_tile<4>, dim3 (g.ncells * (nb - 1) * (nb - 1)), dim3 (256), 0, st, g);
-    hipLaunchKernelGGL (k_gj_tile<5>, dim3 (g.ncells * (nb - 1)), dim3 (256), 0, st, g);
+    const int w = (gj_wide () && nb - s >= 2) ? 2 : 1;
+    for (int q = s; q < s + w; q++)
+    {
+      g.sb = q;
+      hipLaunchKernelGGL (k_gj_diag, dim3 (g.ncells), dim3 (256), 0, st, g);
+      if (w == 2)
+        gj_products (g, q, 1, s, 2, st);
+    }
+    gj_products (g, s, w, 0, nb, st);
     e = hipGetLastError ();
+    s += w;
   }
   return e;
 }
