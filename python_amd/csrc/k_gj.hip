@@ -10,10 +10,10 @@
 //   2. M[S,O] <- D M[S,O]              k_gj_tile<2>
 //   3. M[O,O] <- M[O,O] - M[O,S] M[S,O] k_gj_tile<4>  (M[O,S] still the old values)
 //   4. M[O,S] <- -M[O,S] D             k_gj_tile<5>
-// After the last step M holds M^-1.  Steps normally pivot on two blocks at once (S of 128
-// columns; its diagonal block inverted first by the same sweep over its own 2 x 2 blocks), which
-// halves the HBM traffic of step 3 per flop.  Every product is a 64 x 64 output block with K = 64
-// or 128 on v_mfma_f64_16x16x4f64 (four waves, a 32 x 32 quarter each, operands staged through LDS
+// After the last step M holds M^-1.  Steps normally pivot on four blocks at once (S of 256
+// columns; its diagonal block inverted first by the same sweep over its own blocks, in steps of
+// half the width), which quarters the HBM traffic of step 3 per flop.  Every product is a 64 x 64
+// output block with K = 64 w on v_mfma_f64_16x16x4f64 (four waves, a 32 x 32 quarter each, operands staged through LDS
 // in slabs of 32 k; lane l holds A[l % 16][l / 16] and B[l / 16][l % 16] of a 16 x 4 x 16 step and
 // element e of its result at row l / 16 + 4 e, column l % 16).  The sweep does not pivot: it runs on cells whose A is row diagonally dominant
 // (|a_ii| >= sum_j!=i |a_ij|, which N = I - Q with substochastic Q is).  M is then column
@@ -170,10 +170,10 @@ __device__ __forceinline__ int xcd_item (int bid, int G)
 //   MODE 2: C(S, tj) = D C(S, tj)                (RB = W, CB = 1; one workgroup per tj in O)
 //   MODE 4: C(ti, tj) -= M(ti, S) M(S, tj)       (RB = CB = 1; one per (ti, tj) in O x O)
 //   MODE 5: C(ti, S) = -C(ti, S) D               (RB = 1, CB = W; one per ti in O)
-// MODES 2 and 5 read the blocks they overwrite, so a workgroup computes all W of them before it
-// writes any (W = g.sw is a template parameter).  Operands pass through LDS in slabs of GJ_KC k-rows, the next slab's global loads
-// in flight while the current one runs on the matrix cores; each wave forms a 32 x 32 quarter of
-// every output block.
+// MODES 2 and 5 read the blocks they overwrite, so one workgroup (four waves per block) computes
+// all W of them before it writes any (W = g.sw is a template parameter).  Operands pass through
+// LDS in slabs of GJ_KC k-rows, the next slab's global loads in flight while the current one runs
+// on the matrix cores; each wave forms a 32 x 32 quarter of one output block.
 __device__ __forceinline__ int gj_oblock (int o, int r0, int sb, int w)
 {
   const int b = r0 + o;
@@ -181,11 +181,12 @@ __device__ __forceinline__ int gj_oblock (int o, int r0, int sb, int w)
 }
 
 template <int MODE, int W>
-__global__ __launch_bounds__ (256, (MODE == 4 || W == 1) ? GJ_OCC : 2) void k_gj_tile (GjArgs g)
+__global__ __launch_bounds__ (256 * (MODE == 4 ? 1 : W), (MODE == 4 || W == 1) ? GJ_OCC : (W == 2 ? 2 : 1)) void k_gj_tile (GjArgs g)
 {
   constexpr int RB = MODE == 2 ? W : 1, CB = MODE == 5 ? W : 1;
-  constexpr int NRA = GJ_KC * GJ_NB * RB / 256;   // A elements per thread and slab
-  constexpr int NRB = GJ_KC * GJ_NB * CB / 256;   // B elements per thread and slab
+  constexpr int NT = 256 * RB * CB;               // four waves per output block
+  constexpr int NRA = GJ_KC * GJ_NB * RB / NT;    // A elements per thread and slab
+  constexpr int NRB = GJ_KC * GJ_NB * CB / NT;    // B elements per thread and slab
   constexpr int PA = GJ_NB * RB + 4;              // LDS pitch of the A slab ([k][row])
   constexpr int w = W;                            // g.sw
   const int no = g.nr - w;                        // blocks of O
@@ -219,11 +220,13 @@ __global__ __launch_bounds__ (256, (MODE == 4 || W == 1) ? GJ_OCC : 2) void k_gj
   __shared__ double As[GJ_KC][PA];                // [k][row] of the left operand, sign applied
   __shared__ double Bs[GJ_NB * CB][GJ_PB];        // [col][k] of the right operand
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int rw = (wv >> 1) * 32, cw = (wv & 1) * 32;
+  const int pb = wv >> 2;                         // this wave's output block
+  const int pr = MODE == 2 ? pb : 0, pc = MODE == 5 ? pb : 0;
+  const int rw = ((wv & 3) >> 1) * 32, cw = (wv & 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
 
   // A(k, row) = abase[k n + row] (rows of the RB blocks from ai on), B(col, k) = bbase[col n + k]
-  // (columns of the CB blocks from bj on); this thread's slab elements are idx = tid + 256 e:
+  // (columns of the CB blocks from bj on); this thread's slab elements are idx = tid + NT e:
   // A (k = idx / (64 RB), row = idx % (64 RB)), B (col = idx / GJ_KC, k = idx % GJ_KC)
   const double *abase = m + (size_t) (sb * GJ_NB) * n + ai * GJ_NB;
   const double *bbase = m + (size_t) (bj * GJ_NB) * n + sb * GJ_NB;
@@ -231,36 +234,31 @@ __global__ __launch_bounds__ (256, (MODE == 4 || W == 1) ? GJ_OCC : 2) void k_gj
 #pragma unroll
   for (int e = 0; e < NRA; e++)
   {
-    const int idx = tid + 256 * e;
+    const int idx = tid + NT * e;
     ra[e] = abase[(size_t) (idx / (GJ_NB * RB)) * n + (idx % (GJ_NB * RB))];
   }
 #pragma unroll
   for (int e = 0; e < NRB; e++)
   {
-    const int idx = tid + 256 * e;
+    const int idx = tid + NT * e;
     rb[e] = bbase[(size_t) (idx / GJ_KC) * n + (idx % GJ_KC)];
   }
 
-  gj_d4 acc[RB][CB][2][2];
+  gj_d4 acc[2][2];
 #pragma unroll
-  for (int pr = 0; pr < RB; pr++)
+  for (int bi = 0; bi < 2; bi++)
 #pragma unroll
-    for (int pc = 0; pc < CB; pc++)
-#pragma unroll
-      for (int bi = 0; bi < 2; bi++)
-#pragma unroll
-        for (int bq = 0; bq < 2; bq++)
-        {
-          if (MODE == 4)
-          {
-            // v_mfma_f64_16x16x4f64 keeps element e of lane l at row l/16 + 4 e, column l % 16
-            const double *src =
-                m + (size_t) (bj * GJ_NB + cw + 16 * bq + lr) * n + ai * GJ_NB + rw + 16 * bi + lk;
-            acc[pr][pc][bi][bq] = gj_d4{ src[0], src[4], src[8], src[12] };
-          }
-          else
-            acc[pr][pc][bi][bq] = gj_d4{ 0.0, 0.0, 0.0, 0.0 };
-        }
+    for (int bq = 0; bq < 2; bq++)
+    {
+      if (MODE == 4)
+      {
+        // v_mfma_f64_16x16x4f64 keeps element e of lane l at row l/16 + 4 e, column l % 16
+        const double *src = m + (size_t) (bj * GJ_NB + cw + 16 * bq + lr) * n + ai * GJ_NB + rw + 16 * bi + lk;
+        acc[bi][bq] = gj_d4{ src[0], src[4], src[8], src[12] };
+      }
+      else
+        acc[bi][bq] = gj_d4{ 0.0, 0.0, 0.0, 0.0 };
+    }
 
 #pragma unroll
   for (int kh = 0; kh < K; kh += GJ_KC)
@@ -268,13 +266,13 @@ __global__ __launch_bounds__ (256, (MODE == 4 || W == 1) ? GJ_OCC : 2) void k_gj
 #pragma unroll
     for (int e = 0; e < NRA; e++)
     {
-      const int idx = tid + 256 * e;
+      const int idx = tid + NT * e;
       As[idx / (GJ_NB * RB)][idx % (GJ_NB * RB)] = sgn * ra[e];
     }
 #pragma unroll
     for (int e = 0; e < NRB; e++)
     {
-      const int idx = tid + 256 * e;
+      const int idx = tid + NT * e;
       Bs[idx / GJ_KC][idx % GJ_KC] = rb[e];
     }
     __syncthreads ();
@@ -284,58 +282,45 @@ __global__ __launch_bounds__ (256, (MODE == 4 || W == 1) ? GJ_OCC : 2) void k_gj
 #pragma unroll
       for (int e = 0; e < NRA; e++)
       {
-        const int idx = tid + 256 * e;
+        const int idx = tid + NT * e;
         ra[e] = abase[(size_t) (kh + GJ_KC + idx / (GJ_NB * RB)) * n + (idx % (GJ_NB * RB))];
       }
 #pragma unroll
       for (int e = 0; e < NRB; e++)
       {
-        const int idx = tid + 256 * e;
+        const int idx = tid + NT * e;
         rb[e] = bbase[(size_t) (idx / GJ_KC) * n + kh + GJ_KC + (idx % GJ_KC)];
       }
     }
 #pragma unroll
     for (int k = 0; k < GJ_KC; k += 4)
     {
-      double av[RB][2], bv[CB][2];
+      double av[2], bv[2];
 #pragma unroll
-      for (int pr = 0; pr < RB; pr++)
+      for (int bi = 0; bi < 2; bi++)
+        av[bi] = As[k + lk][GJ_NB * pr + rw + 16 * bi + lr];
 #pragma unroll
-        for (int bi = 0; bi < 2; bi++)
-          av[pr][bi] = As[k + lk][GJ_NB * pr + rw + 16 * bi + lr];
+      for (int bq = 0; bq < 2; bq++)
+        bv[bq] = Bs[GJ_NB * pc + cw + 16 * bq + lr][k + lk];
 #pragma unroll
-      for (int pc = 0; pc < CB; pc++)
+      for (int bi = 0; bi < 2; bi++)
 #pragma unroll
         for (int bq = 0; bq < 2; bq++)
-          bv[pc][bq] = Bs[GJ_NB * pc + cw + 16 * bq + lr][k + lk];
-#pragma unroll
-      for (int pr = 0; pr < RB; pr++)
-#pragma unroll
-        for (int pc = 0; pc < CB; pc++)
-#pragma unroll
-          for (int bi = 0; bi < 2; bi++)
-#pragma unroll
-            for (int bq = 0; bq < 2; bq++)
-              acc[pr][pc][bi][bq] =
-                  __builtin_amdgcn_mfma_f64_16x16x4f64 (av[pr][bi], bv[pc][bq], acc[pr][pc][bi][bq], 0, 0, 0);
+          acc[bi][bq] = __builtin_amdgcn_mfma_f64_16x16x4f64 (av[bi], bv[bq], acc[bi][bq], 0, 0, 0);
     }
     __syncthreads ();
   }
   // every read of the output blocks (MODES 2 and 5 read the blocks they overwrite) is done
 #pragma unroll
-  for (int pr = 0; pr < RB; pr++)
+  for (int bi = 0; bi < 2; bi++)
 #pragma unroll
-    for (int pc = 0; pc < CB; pc++)
+    for (int bq = 0; bq < 2; bq++)
+    {
+      double *dst = m + (size_t) ((bj + pc) * GJ_NB + cw + 16 * bq + lr) * n + (ai + pr) * GJ_NB + rw + 16 * bi + lk;
 #pragma unroll
-      for (int bi = 0; bi < 2; bi++)
-#pragma unroll
-        for (int bq = 0; bq < 2; bq++)
-        {
-          double *dst = m + (size_t) ((bj + pc) * GJ_NB + cw + 16 * bq + lr) * n + (ai + pr) * GJ_NB + rw + 16 * bi + lk;
-#pragma unroll
-          for (int e = 0; e < 4; e++)
-            dst[4 * e] = acc[pr][pc][bi][bq][e];
-        }
+      for (int e = 0; e < 4; e++)
+        dst[4 * e] = acc[bi][bq][e];
+    }
 }
 
 __global__ void k_gj_reset (int *fb)
@@ -362,44 +347,64 @@ hipError_t gj_reset (int *fb, hipStream_t st)
 
 // The products of one sweep step over the blocks [r0, r0 + nr), pivoting on blocks sb .. sb+w-1
 // whose diagonal block is already inverted in place.
-static void gj_products (GjArgs g, int sb, int w, int r0, int nr, hipStream_t st)
+template <int W>
+static void gj_products_w (GjArgs g, int sb, int r0, int nr, hipStream_t st)
 {
-  const int no = nr - w;
+  const int no = nr - W;
   if (no <= 0)
     return;
   g.sb = sb;
-  g.sw = w;
+  g.sw = W;
   g.r0 = r0;
   g.nr = nr;
+  hipLaunchKernelGGL ((k_gj_tile<2, W>), dim3 (g.ncells * no), dim3 (256 * W), 0, st, g);
+  hipLaunchKernelGGL ((k_gj_tile<4, W>), dim3 (g.ncells * no * no), dim3 (256), 0, st, g);
+  hipLaunchKernelGGL ((k_gj_tile<5, W>), dim3 (g.ncells * no), dim3 (256 * W), 0, st, g);
+}
+
+static void gj_products (const GjArgs &g, int sb, int w, int r0, int nr, hipStream_t st)
+{
+  if (w == 1)
+    gj_products_w<1> (g, sb, r0, nr, st);
+  else if (w == 2)
+    gj_products_w<2> (g, sb, r0, nr, st);
+  else
+    gj_products_w<4> (g, sb, r0, nr, st);
+}
+
+// M[S,S] <- M[S,S]^-1 for the w blocks S = s .. s+w-1 (w = 1, 2 or 4): one block in LDS, else
+// the sweep of S's own blocks in half-width steps
+static void gj_invert_diag (GjArgs g, int s, int w, hipStream_t st)
+{
   if (w == 1)
   {
-    hipLaunchKernelGGL ((k_gj_tile<2, 1>), dim3 (g.ncells * no), dim3 (256), 0, st, g);
-    hipLaunchKernelGGL ((k_gj_tile<4, 1>), dim3 (g.ncells * no * no), dim3 (256), 0, st, g);
-    hipLaunchKernelGGL ((k_gj_tile<5, 1>), dim3 (g.ncells * no), dim3 (256), 0, st, g);
+    g.sb = s;
+    hipLaunchKernelGGL (k_gj_diag, dim3 (g.ncells), dim3 (256), 0, st, g);
+    return;
   }
-  else
+  for (int q = s; q < s + w; q += w / 2)
   {
-    hipLaunchKernelGGL ((k_gj_tile<2, 2>), dim3 (g.ncells * no), dim3 (256), 0, st, g);
-    hipLaunchKernelGGL ((k_gj_tile<4, 2>), dim3 (g.ncells * no * no), dim3 (256), 0, st, g);
-    hipLaunchKernelGGL ((k_gj_tile<5, 2>), dim3 (g.ncells * no), dim3 (256), 0, st, g);
+    gj_invert_diag (g, q, w / 2, st);
+    gj_products (g, q, w / 2, s, w, st);
   }
 }
 
-// Steps of 128 columns (two blocks) halve the trailing update's HBM traffic per flop: each reads
-// and writes every other block once per 128 columns instead of once per 64.
-// HIPLA_GJ_STEP=64 in the environment keeps 64-column steps.
-static bool gj_wide ()
+// Steps of up to 256 columns (four blocks) cut the trailing update's HBM traffic per flop: each
+// step reads and writes every other block once.  HIPLA_GJ_STEP=64 or 128 in the environment
+// caps the step width.
+static int gj_max_width ()
 {
-  static const bool on = [] {
+  static const int wmax = [] {
     const char *e = getenv ("HIPLA_GJ_STEP");
-    return !(e && std::atoi (e) == 64);
+    const int v = e ? std::atoi (e) : 256;
+    return v <= 64 ? 1 : v <= 128 ? 2 : 4;
   }();
-  return on;
+  return wmax;
 }
 
-// every launch of one size class: prepare, then the sweep.  A two-block step first inverts its
-// 128 x 128 diagonal block in place by a sweep of that block's own 2 x 2 blocks, then forms the
-// products of the whole step with K = 128.
+// every launch of one size class: prepare, then the sweep.  A step of w > 1 blocks first inverts
+// its diagonal block in place by a sweep of that block's own blocks, then forms the products of
+// the whole step with K = 64 w.
 hipError_t gj_enqueue_class (const GjArgs &g0, hipStream_t st)
 {
   if (g0.ncells <= 0)
@@ -414,14 +419,10 @@ hipError_t gj_enqueue_class (const GjArgs &g0, hipStream_t st)
   hipError_t e = hipGetLastError ();
   for (int s = 0; s < nb && e == hipSuccess;)
   {
-    const int w = (gj_wide () && nb - s >= 2) ? 2 : 1;
-    for (int q = s; q < s + w; q++)
-    {
-      g.sb = q;
-      hipLaunchKernelGGL (k_gj_diag, dim3 (g.ncells), dim3 (256), 0, st, g);
-      if (w == 2)
-        gj_products (g, q, 1, s, 2, st);
-    }
+    int w = gj_max_width ();
+    while (w > nb - s)
+      w /= 2;
+    gj_invert_diag (g, s, w, st);
     gj_products (g, s, w, 0, nb, st);
     e = hipGetLastError ();
     s += w;
