@@ -55,26 +55,42 @@ __global__ __launch_bounds__ (256) void k_gj_prepare (GjArgs g)
     s_bad = 0;
   __syncthreads ();
   int bad = 0;
-  for (int r = wv; r < n; r += 4)
+  // eight rows per pass (n is a multiple of 64, so of 32 rows per wave), so that eight loads are
+  // in flight per column step and the eight row reductions interleave
+  constexpr int PR = 8;
+  for (int r0 = wv * PR; r0 < n; r0 += 4 * PR)
   {
-    double off = 0.0, dg = 0.0;
-    const double *ar = a + (size_t) r * n;
-    double *mr = m + (size_t) r * n;
+    double off[PR], dg[PR];
+#pragma unroll
+    for (int q = 0; q < PR; q++)
+      off[q] = dg[q] = 0.0;
     for (int c = lane; c < n; c += 64)
     {
-      const double v = ar[c];
-      mr[c] = v;
-      if (c == r)
-        dg = fabs (v);
-      else
-        off += fabs (v);
+      double v[PR];
+#pragma unroll
+      for (int q = 0; q < PR; q++)
+        v[q] = a[(size_t) (r0 + q) * n + c];
+#pragma unroll
+      for (int q = 0; q < PR; q++)
+      {
+        m[(size_t) (r0 + q) * n + c] = v[q];
+        if (c == r0 + q)
+          dg[q] = fabs (v[q]);
+        else
+          off[q] += fabs (v[q]);
+      }
     }
+#pragma unroll
     for (int o = 32; o > 0; o >>= 1)
-    {
-      off += __shfl_xor (off, o);
-      dg += __shfl_xor (dg, o);
-    }
-    bad |= !(dg >= off);          // NaN counts as not dominant
+#pragma unroll
+      for (int q = 0; q < PR; q++)
+      {
+        off[q] += __shfl_xor (off[q], o);
+        dg[q] += __shfl_xor (dg[q], o);
+      }
+#pragma unroll
+    for (int q = 0; q < PR; q++)
+      bad |= !(dg[q] >= off[q]);  // NaN counts as not dominant
   }
   if (lane == 0 && bad)
     atomicOr (&s_bad, 1);
@@ -91,7 +107,11 @@ __global__ __launch_bounds__ (256) void k_gj_prepare (GjArgs g)
   }
 }
 
-// ---- step 1: D = M[S,S]^-1 by the unblocked sweep (no pivoting) in LDS ----
+// ---- step 1: D = M[S,S]^-1 by the unblocked sweep (no pivoting) ----
+// Thread (column j = lane, rows r0 .. r0+15 with r0 = 16 wave) keeps its 16 elements in registers.
+// Step k publishes column k (its owners) and row k (the wave holding that row) to LDS slots of the
+// step's parity, so one barrier per step suffices: a thread can only overwrite a slot two steps
+// later, after every thread has passed the barrier of the step in between.
 __global__ __launch_bounds__ (256) void k_gj_diag (GjArgs g)
 {
   const int cell = g.list[blockIdx.x];
@@ -100,19 +120,33 @@ __global__ __launch_bounds__ (256) void k_gj_diag (GjArgs g)
   const int n = g.nsz;
   const int k0 = g.sb * GJ_NB;
   double *m = g.M + g.offA[cell];
-  __shared__ double B[GJ_NB][GJ_NB + 1];   // B[col][row]
+  __shared__ double s_col[2][GJ_NB];       // column k (by row), by step parity
+  __shared__ double s_row[2][GJ_NB];       // row k (by column)
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < GJ_NB * GJ_NB; idx += 256)
+  const int j = tid & 63;                  // this thread's column
+  const int wv = tid >> 6;
+  const int r0 = wv * 16;                  // and its 16 rows
+  double b[16];
   {
-    const int c = idx >> 6, r = idx & 63;
-    B[c][r] = m[(size_t) (k0 + c) * n + k0 + r];
+    const double *src = m + (size_t) (k0 + j) * n + k0 + r0;
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+      b[i] = src[i];
   }
-  __syncthreads ();
-  const int j = tid & 63;            // this thread's column
-  const int r0 = (tid >> 6) * 16;    // and its 16 rows
+#pragma unroll
   for (int k = 0; k < GJ_NB; k++)
   {
-    const double piv = B[k][k];
+    const int buf = k & 1;
+    if (j == k)
+    {
+#pragma unroll
+      for (int i = 0; i < 16; i++)
+        s_col[buf][r0 + i] = b[i];
+    }
+    if (wv == k / 16)
+      s_row[buf][j] = b[k % 16];
+    __syncthreads ();
+    const double piv = s_col[buf][k];
     if (piv == 0.0)
     {
       // an exact zero pivot: M has a zero column (the block's Schur complement of a column
@@ -125,32 +159,28 @@ __global__ __launch_bounds__ (256) void k_gj_diag (GjArgs g)
       return;
     }
     const double inv = 1.0 / piv;
-    const double u = B[j][k];        // row k, column j
     double f[16];
 #pragma unroll
     for (int i = 0; i < 16; i++)
-      f[i] = B[k][r0 + i];           // column k, this thread's rows
-    __syncthreads ();
+      f[i] = s_col[buf][r0 + i];           // column k, this thread's rows
     if (j == k)
     {
 #pragma unroll
       for (int i = 0; i < 16; i++)
-        B[k][r0 + i] = (r0 + i == k) ? inv : -f[i] * inv;
+        b[i] = (r0 + i == k) ? inv : -f[i] * inv;
     }
     else
     {
-      const double uj = u * inv;
+      const double uj = s_row[buf][j] * inv;   // row k, column j
 #pragma unroll
       for (int i = 0; i < 16; i++)
-        B[j][r0 + i] = (r0 + i == k) ? uj : fma (-f[i], uj, B[j][r0 + i]);
+        b[i] = (r0 + i == k) ? uj : fma (-f[i], uj, b[i]);
     }
-    __syncthreads ();
   }
-  for (int idx = tid; idx < GJ_NB * GJ_NB; idx += 256)
-  {
-    const int c = idx >> 6, r = idx & 63;
-    m[(size_t) (k0 + c) * n + k0 + r] = B[c][r];
-  }
+  double *dst = m + (size_t) (k0 + j) * n + k0 + r0;
+#pragma unroll
+  for (int i = 0; i < 16; i++)
+    dst[i] = b[i];
 }
 
 // Workgroups are dispatched to the eight XCDs round robin; number them so that consecutive tiles
