@@ -311,9 +311,11 @@ def bench_c4(args):
 
 
 def bench_c5(args):
-    """T_e gradient: 10,000 cells of n = 64 whose rates depend on T_e, seed 5.  A step is the torch
-    assembly of A(T_e), the batched FP64 solve (factors kept), the adjoint A^T lambda = dL/dx and
-    the chain to dL/dT_e; the gradient is checked against the FP64 oracle on a sample."""
+    """T_e gradient: 10,000 cells of n = 64 whose rates depend on T_e, seed 5.  A step is the GPU
+    assembly of A(T_e) (hipla_te_assemble), the batched FP64 solve (factors kept), the adjoint
+    A^T lambda = dL/dx and dL/dT_e straight from lambda (hipla_te_grad), behind
+    torch.autograd; the same step through torch's assembly and chain rule over the dense dL/dA is
+    timed beside it.  The gradient is checked against the FP64 oracle on a sample."""
     import torch
     import oracle
     from python_amd.adjoint import IonTe, loss_and_grad
@@ -327,6 +329,11 @@ def bench_c5(args):
     T = torch.from_numpy(d["te"]).cuda()
     wt = torch.from_numpy(w).cuda()
     res = {}
+
+    def step_torch():
+        res["out"] = loss_and_grad(model, plan, T, wt, fused=False)
+
+    ms_torch = _timed_ms(step_torch, args.steps, args.warmup)
 
     def step():
         res["out"] = loss_and_grad(model, plan, T, wt)
@@ -349,6 +356,8 @@ def bench_c5(args):
         "value": round(ncells / (ms * 1e-3), 1), "ms_per_step": round(ms, 4), "n_gpus": 1, "dtype": "f64",
         "data": "synthetic (T_e-dependent ionization matrices, seed 5)",
         "config": {"workload": f"te_gradient: {ncells} cells, n = 64, loss = sum_i w_i x_i"},
+        "torch_assembly_and_chain_rule": {"cells_per_s": round(ncells / (ms_torch * 1e-3), 1),
+                                          "ms_per_step": round(ms_torch, 4)},
         "gradient_error_over_bound_max": round(worst, 6),
         "gradient_bound": "1e-8 |g| + 1e-10 sum|lambda||dA||x| against the oracle on 200 sampled cells"})
 

@@ -134,6 +134,24 @@ int hipla_invert (hipla_plan plan, const double *dA, double *dInv, int *dcodes, 
 int hipla_adjoint (hipla_plan plan, const double *dX, const double *dG, double *dLambda, double *dGradA,
                    void *stream);
 
+/* The T_e gradient workload's rate model on the GPU (no counterpart in the reference, which finds
+ * T_e by a root find around the solve, source/ionization.c:486).  ncells matrices of order
+ * n <= 64, flat (cell c's A at c n^2, its b at c n).  Stage q joins rows r = dRow[q] and r + 1
+ * (0 <= r < n - 1): A[r+1][r] = C T^0.5 exp(-E / kT), A[r][r+1] = a0 (T / 1e4)^-0.7, with C, E,
+ * a0 of shape [ncells][nstage] and T per cell; each diagonal is minus its column sum; row
+ * dConsS[e] is replaced by ones over the dConsM[e] ions from it and b there is dConsAb[e] (the
+ * abundance rows of populate_ion_rate_matrix, source/matrix_ion.c:633-655); every other b is 0. */
+int hipla_te_assemble (int ncells, int n, int nstage, const int *dRow, const double *dC, const double *dE,
+                       const double *dA0, int ncons, const int *dConsS, const int *dConsM, const double *dConsAb,
+                       const double *dT, double *dA, double *dB, void *stream);
+
+/* dL/dT_e = -lambda^T (dA/dT) x per cell into dGradT[ncells], for the same model, from the
+ * solution x and the adjoint's lambda (hipla_adjoint with dGradA NULL): the dense dL/dA is never
+ * formed.  Abundance rows carry no T dependence. */
+int hipla_te_grad (int ncells, int n, int nstage, const int *dRow, const double *dC, const double *dE,
+                   const double *dA0, int ncons, const int *dConsS, const double *dT, const double *dX,
+                   const double *dLambda, double *dGradT, void *stream);
+
 /* Macro-atom emissivities without forming N^-1 (needs HIPLA_FLAG_KEEP_FACTORS, HIPLA_MODE_FP64).
  * Replaces invert_matrix + B = N^-1 R (source/macro_accelerate.c:315-334) + the contraction of
  * source/macro_gen_f.c:490-506 by one factorisation and one transposed solve per cell: with N the

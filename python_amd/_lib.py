@@ -68,6 +68,10 @@ SIGNATURES = {
     "hipla_plan_describe": (c_int, [c_void_p, ctypes.c_char_p, c_int]),
     "hipla_plan_set_variant": (c_int, [c_void_p, c_int, c_int]),
     "hipla_plan_time_groups": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, _dp, c_int]),
+    "hipla_te_assemble": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
+                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "hipla_te_grad": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
+                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
 }
 
 

@@ -49,7 +49,29 @@ def test_adjoint_lambda_matches_oracle(hip, oracle_lib):
         assert np.allclose(ga, -np.outer(got, Xh[b.offV[c]:b.offV[c] + n]), rtol=0, atol=0)
 
 
-def test_te_gradient_fd_and_reference(hip, oracle_lib):
+def test_te_assembly_on_gpu_matches_numpy(hip):
+    """hipla_te_assemble against the numpy restatement of the rate model (te_assemble): the same
+    entries, up to the last bits of exp and pow (1e-14 relative), the abundance rows and b exact."""
+    import torch
+    from python_amd.generators import te_assemble
+    d, model, w = _model()
+    n, ncells = d["A"].shape[1], d["A"].shape[0]
+    T = torch.from_numpy(d["te"]).cuda()
+    A, b = model.assemble_gpu(T)
+    A = A.cpu().numpy().reshape(ncells, n, n)
+    b = b.cpu().numpy().reshape(ncells, n)
+    Aref, _, bref = te_assemble(n, d["blocks"], d["params"], d["te"])
+    assert np.array_equal(b, bref)
+    assert np.array_equal(A == 0.0, Aref == 0.0)
+    assert np.allclose(A, Aref, rtol=1e-14, atol=0.0)
+    for (s, m, ab) in model.cons:
+        assert np.array_equal(A[:, s, :], Aref[:, s, :])
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_te_gradient_fd_and_reference(hip, oracle_lib, fused):
+    """dL/dT_e through the adjoint: fused (GPU assembly, hipla_te_grad from lambda) and through
+    torch's chain rule over the dense dL/dA (torch assembly)."""
     import torch
     from python_amd.adjoint import loss_and_grad
     from python_amd.generators import te_assemble
@@ -58,7 +80,7 @@ def test_te_gradient_fd_and_reference(hip, oracle_lib):
     plan = model.plan()
     T = torch.from_numpy(d["te"]).cuda()
     wt = torch.from_numpy(w).cuda()
-    L, g, codes = loss_and_grad(model, plan, T, wt)
+    L, g, codes = loss_and_grad(model, plan, T, wt, fused=fused)
     L, g = L.cpu().numpy(), g.cpu().numpy()
 
     # FP64 reference gradient with the oracle, -lambda^T (dA/dT) x, on every cell.  The sum has
