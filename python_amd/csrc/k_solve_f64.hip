@@ -269,10 +269,9 @@ struct WaveF64Layout
   static constexpr int SP = NC | 1;              // staging pitch (odd: no bank conflicts)
   static constexpr int BR = NC + 2;              // broadcast row: [t] = U(k, k+t), [NC] = y_k
   static constexpr int XS = NC + 8;              // x, then +0 up to the next group of eight
-  static constexpr int LDL = NC + 1;             // KEEP: multipliers of one row
   static constexpr int base (bool keep)
   {
-    return ((SR * SP + 1) & ~1) + BR + XS + (keep ? NC * LDL + NC / 2 : 0);
+    return ((SR * SP + 1) & ~1) + BR + XS + (keep ? NC / 2 : 0);
   }
 };
 
@@ -319,15 +318,14 @@ __global__ __launch_bounds__ (64 * WPB, NC == 64 ? 3 : 4) void k_wave_f64 (Solve
 {
   extern __shared__ double smem[];
   using Lay = WaveF64Layout<NC>;
-  constexpr int SR = Lay::SR, SP = Lay::SP, LDL = Lay::LDL;
+  constexpr int SR = Lay::SR, SP = Lay::SP;
   constexpr int PER_WAVE = (Lay::base (KEEP) + 1) & ~1;
   const int wv = uni (threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int neff = list_len (s);
   double *stg = smem + wv * PER_WAVE;
   double *brow = stg + ((SR * SP + 1) & ~1);
   double *xs = brow + Lay::BR;
-  double *Ls = xs + Lay::XS;                          // KEEP: Ls[row * LDL + k] = multiplier
-  int *pvs = reinterpret_cast<int *> (Ls + NC * LDL); // KEEP: GSL's ipiv
+  int *pvs = reinterpret_cast<int *> (xs + Lay::XS);  // KEEP: GSL's ipiv
   for (int item = blockIdx.x * WPB + wv; item < neff; item += gridDim.x * WPB)
   {
     wave_lds_fence ();
@@ -400,8 +398,10 @@ __global__ __launch_bounds__ (64 * WPB, NC == 64 ? 3 : 4) void k_wave_f64 (Solve
       if (alive)
       {
         const double l = gsl_multiplier (a[0], piv);
-        if (KEEP)
-          Ls[lane * LDL + k] = l;
+        // KEEP: multipliers go to row `lane` (the physical row) of the factor output; the rows
+        // move to their GSL positions after the last step
+        if (KEEP && s.LU)
+          s.LU[s.offA[cell] + (size_t) lane * n + k] = l;
         bb = bb - l * brow[NC];
 #pragma unroll
         for (int g = 0; g < NC; g += 8)
@@ -426,16 +426,34 @@ __global__ __launch_bounds__ (64 * WPB, NC == 64 ? 3 : 4) void k_wave_f64 (Solve
     }
     wave_lds_fence ();
 
-    if (KEEP && s.LU && in)
+    if (KEEP && s.LU)
     {
-      // row `lane` became pivot at step pos: LU row pos = [L multipliers | U row]
-      double *out = s.LU + s.offA[cell] + (size_t) pos * n;
-      for (int j = 0; j < pos; j++)
-        out[j] = Ls[lane * LDL + j];
+      // row `lane` became pivot at step pos: LU row pos = [L multipliers | U row].  Its
+      // multipliers sit in row `lane`; they move to row pos eight columns at a time, each
+      // chunk read by every lane before any lane writes it (the stores wait on the loads they
+      // store), then the lane writes its U row behind them.
+      double *lu = s.LU + s.offA[cell];
+      __builtin_amdgcn_fence (__ATOMIC_ACQ_REL, "workgroup");   // this wave's multiplier stores
+      for (int c0 = 0; c0 < n; c0 += 8)
+      {
+        double v[8];
 #pragma unroll
-      for (int t = 0; t < NC; t++)
-        if (t < n - pos)
-          out[pos + t] = a[t];
+        for (int e = 0; e < 8; e++)
+          v[e] = (in && c0 + e < pos) ? lu[(size_t) lane * n + c0 + e] : 0.0;
+        __builtin_amdgcn_fence (__ATOMIC_ACQ_REL, "workgroup");
+#pragma unroll
+        for (int e = 0; e < 8; e++)
+          if (in && c0 + e < pos)
+            lu[(size_t) pos * n + c0 + e] = v[e];
+      }
+      if (in)
+      {
+        double *out = lu + (size_t) pos * n;
+#pragma unroll
+        for (int t = 0; t < NC; t++)
+          if (t < n - pos)
+            out[pos + t] = a[t];
+      }
     }
     if (KEEP && s.piv)
       for (int t = lane; t < n; t += 64)
