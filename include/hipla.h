@@ -122,6 +122,8 @@ int hipla_solve (hipla_plan plan, const double *dA, const double *dB, double *dX
  * inverse on the matrix cores: in HIPLA_MODE_FP64 with FP64 MFMA block products, in
  * HIPLA_MODE_MIXED with split BF16x3 MFMA products in FP32 followed by two FP64 residual
  * refinement steps.  Every other cell runs the GSL-order FP64 LU inverse with partial pivoting.
+ * Environment: HIPLA_INVERT_SWEEP=0 sends every cell to the pivoting inverse; HIPLA_GJ_STEP=64
+ * or 128 caps the FP64 sweep's step width (default 256 columns). 
  * All are within 1e-8 relative Frobenius of the reference's gsl_linalg_LU_invx on the
  * workloads tested; none is in its operation order. */
 int hipla_invert (hipla_plan plan, const double *dA, double *dInv, int *dcodes, hipla_cell_stats *dstats,
