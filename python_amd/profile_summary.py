@@ -48,6 +48,41 @@ def _counters(path, counter):
     return per
 
 
+def _counter_seq(path, counter):
+    """(kernel name, value) of one counter for every dispatch, in dispatch order."""
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] == counter:
+                rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+    rows.sort()
+    return [(nm, v) for _, nm, v in rows]
+
+
+def _trace_seq(trace):
+    """(kernel name, duration ns) of every dispatch of a kernel trace, in start order."""
+    rows = sorted(trace, key=lambda r: int(r["Start_Timestamp"]))
+    return [(r["Kernel_Name"], int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) for r in rows]
+
+
+def _unit_values(seq, parts, is_part):
+    """Per part, the values of the dispatches that form the unit: a dispatch of the first part
+    followed directly by one of each further part (a factor kernel and its back substitution),
+    so that another unit launching the same factor kernel with a different back substitution is
+    not counted.  A one-part unit takes every dispatch of its kernel."""
+    out = [[] for _ in parts]
+    i = 0
+    while i < len(seq):
+        if is_part(seq[i][0], parts[0]) and i + len(parts) <= len(seq) and all(
+                is_part(seq[i + j][0], parts[j]) for j in range(1, len(parts))):
+            for j in range(len(parts)):
+                out[j].append(seq[i + j][1])
+            i += len(parts)
+        else:
+            i += 1
+    return out
+
+
 def main(out=None):
     out = out or os.path.join(ROOT, "gpurun_out", "rocprof")
     rl = _roofline(out)
@@ -63,10 +98,8 @@ def main(out=None):
     def is_part(name, part):
         return name.startswith("void " + part + "(") or name.startswith(part + "(")
 
-    per_part = []
-    for part in parts:
-        d = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in trace if is_part(r["Kernel_Name"], part)]
-        per_part.append(d)
+    tseq = _trace_seq(trace)
+    per_part = _unit_values(tseq, parts, is_part)
     dom_d = per_part[0]
     os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
     short = _short(parts[0])
@@ -105,12 +138,10 @@ def main(out=None):
     kib = calib_bytes / 1024.0
     ff = max(f_cal) / kib
     fw = max(w_cal) / kib
-    f_kib = w_kib = 0.0
-    for part in parts:
-        _, fd = pick(fetch, lambda k: is_part(k, part))
-        _, wd = pick(write, lambda k: is_part(k, part))
-        f_kib += sum(fd) / len(fd)
-        w_kib += sum(wd) / len(wd)
+    fseq = _counter_seq(os.path.join(out, "fetch", "bench_counter_collection.csv"), "FETCH_SIZE")
+    wseq = _counter_seq(os.path.join(out, "write", "bench_counter_collection.csv"), "WRITE_SIZE")
+    f_kib = sum(sum(d) / len(d) for d in _unit_values(fseq, parts, is_part))
+    w_kib = sum(sum(d) / len(d) for d in _unit_values(wseq, parts, is_part))
     f_dom = [f_kib]
     w_dom = [w_kib]
     fetch_b = f_kib * 1024.0 / ff
@@ -136,13 +167,13 @@ def main(out=None):
           f"algorithmic ({traffic / roof['algorithmic_bytes_per_launch']:.2f}x)"]
     with open(os.path.join(ROOT, "profiles", f"{short}_uneven10k_hbm.txt"), "w") as f:
         f.write("\n".join(hl) + "\n")
-    cl = _class_table(rl, trace, fetch, write, ff, fw, is_part)
+    cl = _class_table(rl, tseq, fseq, wseq, ff, fw, is_part)
     with open(os.path.join(ROOT, "profiles", "size_classes_uneven10k.txt"), "w") as f:
         f.write("\n".join(cl) + "\n")
     print("\n".join(lines + [""] + hl + [""] + cl))
 
 
-def _class_table(rl, trace, fetch, write, ff, fw, is_part):
+def _class_table(rl, tseq, fseq, wseq, ff, fw, is_part):
     """Every FP64 size class of the bench workload: rocprofv3 average duration of its kernels,
     algorithmic bytes, achieved GB/s and the calibrated FETCH_SIZE + WRITE_SIZE bytes."""
     out = ["# Size classes of the bench workload (uneven 10k-cell batch), each launched in isolation",
@@ -155,21 +186,20 @@ def _class_table(rl, trace, fetch, write, ff, fw, is_part):
            f"{'HBM MB':>8s} {'x alg':>6s}"]
     for c in rl.get("classes", []):
         parts = [x.strip() for x in c["kernel"].split(" + ")]
+        # classes sharing a unit (the same kernels): the dispatches of this class's count
+        same = [x for x in rl["classes"] if x["kernel"] == c["kernel"]]
         us = 0.0
         hbm = 0.0
-        for part in parts:
-            d = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in trace if is_part(r["Kernel_Name"], part)]
-            # classes sharing a kernel instantiation: the dispatches of this class's count
-            same = [x for x in rl["classes"] if x["kernel"] == c["kernel"]]
+        tv = _unit_values(tseq, parts, is_part)
+        fvs = _unit_values(fseq, parts, is_part)
+        wvs = _unit_values(wseq, parts, is_part)
+        for j in range(len(parts)):
+            d, fv, wv = tv[j], fvs[j], wvs[j]
             if len(same) > 1:
-                d = d[same.index(c)::len(same)]
+                k = same.index(c)
+                d, fv, wv = d[k::len(same)], fv[k::len(same)], wv[k::len(same)]
             us += sum(d) / len(d) / 1e3 if d else float("nan")
-            fk = [v for k, v in fetch.items() if is_part(k, part)]
-            wk = [v for k, v in write.items() if is_part(k, part)]
-            if fk and wk:
-                fv, wv = fk[0], wk[0]
-                if len(same) > 1:
-                    fv, wv = fv[same.index(c)::len(same)], wv[same.index(c)::len(same)]
+            if fv and wv:
                 hbm += (sum(fv) / len(fv)) * 1024.0 / ff + (sum(wv) / len(wv)) * 1024.0 / fw
         alg = c.get("bytes", 0.0)
         gbs = alg / (us * 1e-6) / 1e9 if us > 0 else float("nan")
