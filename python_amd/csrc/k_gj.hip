@@ -4,18 +4,20 @@
 //
 // The row-major input A, read as a column-major matrix, is M = A^T; the inverse of M written back
 // in the same place, read row-major again, is A^-1.  M is inverted in place by the blocked
-// Gauss-Jordan sweep with 64 x 64 blocks, every cell of a size class in lock step.  For the
-// diagonal block S of step s (O = every other block index):
-//   1. M[S,S] <- D = M[S,S]^-1        k_gj_diag: unblocked sweep of the 64 x 64 block in LDS
-//   2. M[S,O] <- D M[S,O]              k_gj_tile<2>
-//   3. M[O,O] <- M[O,O] - M[O,S] M[S,O] k_gj_tile<4>  (M[O,S] still the old values)
-//   4. M[O,S] <- -M[O,S] D             k_gj_tile<5>
-// After the last step M holds M^-1.  Steps normally pivot on four blocks at once (S of 256
-// columns; its diagonal block inverted first by the same sweep over its own blocks, in steps of
-// half the width), which quarters the HBM traffic of step 3 per flop.  Every product is a 64 x 64
-// output block with K = 64 w on v_mfma_f64_16x16x4f64 (four waves, a 32 x 32 quarter each, operands staged through LDS
-// in slabs of 32 k; lane l holds A[l % 16][l / 16] and B[l / 16][l % 16] of a 16 x 4 x 16 step and
-// element e of its result at row l / 16 + 4 e, column l % 16).  The sweep does not pivot: it runs on cells whose A is row diagonally dominant
+// Gauss-Jordan sweep over 64 x 64 blocks, every cell of a size class in lock step.  A step pivots
+// on the w diagonal blocks S (w = 4, or fewer at the end of an order that is not a multiple of
+// 256), O = every other block:
+//   1. M[S,S] <- D = M[S,S]^-1        the same sweep over S's own blocks in steps of w/2, down to
+//                                     k_gj_diag (one 64 x 64 block, in registers)
+//   2. M[S,O] <- D M[S,O]              k_gj_tile<2, w>
+//   3. M[O,O] <- M[O,O] - M[O,S] M[S,O] k_gj_tile<4, w>  (M[O,S] still the old values)
+//   4. M[O,S] <- -M[O,S] D             k_gj_tile<5, w>
+// After the last step M holds M^-1.  Step 3 reads and writes every other block once per step, so
+// its HBM traffic per flop falls as 1/w.  Every product is a 64 x 64 output block with K = 64 w on
+// v_mfma_f64_16x16x4f64 (four waves, a 32 x 32 quarter each, operands staged through LDS in slabs
+// of 32 k; lane l holds A[l % 16][l / 16] and B[l / 16][l % 16] of a 16 x 4 x 16 step and element
+// e of its result at row l / 16 + 4 e, column l % 16).  The sweep does not pivot: it runs on cells
+// whose A is row diagonally dominant
 // (|a_ii| >= sum_j!=i |a_ij|, which N = I - Q with substochastic Q is).  M is then column
 // diagonally dominant, every Schur complement stays so, and partial pivoting would never swap a
 // row; k_gj_prepare hands every other cell to the GSL-order pivoting kernel (k_invert_f64).  An

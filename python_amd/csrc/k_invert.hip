@@ -424,8 +424,23 @@ int plan_enqueue_invert (hipla_plan p, const double *dA, double *dInv, int *dcod
       g.fb = p->d_gj_fb;
       HIPLA_CHECK (gjx_enqueue_class (g, GX_REFINE_STEPS, st));
     }
+    // FP64 sweep: the class with the most work on the caller's stream, the others one after
+    // another on the fork stream beside it (their launches hold fewer tiles than the chip has
+    // slots); the caller's stream waits for them before the hand-back
+    size_t kbig = 0;
+    for (size_t k = 1; k < p->gj_n.size (); k++)
+      if ((double) p->gj_n[k] * p->gj_n[k] * p->gj_n[k] * p->gj_count[k] >
+          (double) p->gj_n[kbig] * p->gj_n[kbig] * p->gj_n[kbig] * p->gj_count[kbig])
+        kbig = k;
+    const bool forkc = p->mode != HIPLA_MODE_MIXED && p->gj_n.size () > 1;
+    if (forkc)
+    {
+      HIPLA_CHECK (hipEventRecord (p->ev_fork, st));
+      HIPLA_CHECK (hipStreamWaitEvent (p->fork_st, p->ev_fork, 0));
+    }
     for (size_t k = 0; k < p->gj_n.size () && p->mode != HIPLA_MODE_MIXED; k++)
     {
+      hipStream_t cs = (forkc && k != kbig) ? p->fork_st : st;
       GjArgs g;
       g.offA = p->d_offA;
       g.list = p->d_gj_list + p->gj_first[k];
@@ -437,7 +452,12 @@ int plan_enqueue_invert (hipla_plan p, const double *dA, double *dInv, int *dcod
       g.codes = dcodes;
       g.flag = p->d_gj_flag;
       g.fb = p->d_gj_fb;
-      HIPLA_CHECK (gj_enqueue_class (g, st));
+      HIPLA_CHECK (gj_enqueue_class (g, cs));
+    }
+    if (forkc)
+    {
+      HIPLA_CHECK (hipEventRecord (p->ev_join, p->fork_st));
+      HIPLA_CHECK (hipStreamWaitEvent (st, p->ev_join, 0));
     }
   }
   if (p->inv_nrest > 0)
