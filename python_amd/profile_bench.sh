@@ -3,11 +3,11 @@
 #   1. rocprofv3 kernel trace + per-kernel stats of `bench.py --roofline-only --no-cpu`
 #   2. HBM bytes read per dispatch (FETCH_SIZE) of the same command
 #   3. HBM bytes written per dispatch (WRITE_SIZE) of the same command
-# Raw output goes to $OUT (default gpurun_out/rocprof under the repository root).  Run from the
+# Raw output goes to $OUT (default profile_out/rocprof under the repository root).  Run from the
 # repository root on a machine with the GPU; summarise with python_amd/profile_summary.py.
 set -e
 R=$(cd "$(dirname "$0")/.." && pwd)
-OUT=${OUT:-$R/gpurun_out/rocprof}
+OUT=${OUT:-$R/profile_out/rocprof}
 mkdir -p "$OUT"
 cd /tmp
 export TMPDIR=/tmp

@@ -3,11 +3,11 @@
 #   1. rocprofv3 kernel trace + per-kernel stats of `bench.py --config c4 --steps 1` (both plans)
 #   2. MFMA counters per dispatch of the FP64 plan on a tenth of the batch (--quick):
 #      SQ_VALU_MFMA_BUSY_CYCLES, GRBM_GUI_ACTIVE, SQ_INSTS_VALU_MFMA_MOPS_F64, _MOPS_BF16
-# Raw output goes to $OUT (default gpurun_out/prof_c4 under the repository root); summarise with
+# Raw output goes to $OUT (default profile_out/prof_c4 under the repository root); summarise with
 # python -m python_amd.profile_summary c4 [OUT].
 set -e
 R=$(cd "$(dirname "$0")/.." && pwd)
-OUT=${OUT:-$R/gpurun_out/prof_c4}
+OUT=${OUT:-$R/profile_out/prof_c4}
 mkdir -p "$OUT"
 cd /tmp
 export TMPDIR=/tmp

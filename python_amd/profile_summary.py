@@ -1,6 +1,6 @@
 """Summarise the rocprofv3 output of python_amd/profile_bench.sh into profiles/.
 
-    python -m python_amd.profile_summary [OUT_DIR]      (default gpurun_out/rocprof)
+    python -m python_amd.profile_summary [OUT_DIR]      (default profile_out/rocprof)
 
 Reads the kernel statistics and kernel trace of `bench.py --roofline-only --no-cpu`, its
 FETCH_SIZE and WRITE_SIZE counter runs and the roofline object the command printed, and writes
@@ -84,7 +84,7 @@ def _unit_values(seq, parts, is_part):
 
 
 def main(out=None):
-    out = out or os.path.join(ROOT, "gpurun_out", "rocprof")
+    out = out or os.path.join(ROOT, "profile_out", "rocprof")
     rl = _roofline(out)
     roof = rl["roofline"]
     dom = roof["kernel"]
@@ -218,7 +218,7 @@ def main_c4(out=None):
     """profiles/gj_sweep_c4_trace.txt (per-kernel statistics of both plans over one warm-up and one
     timed inverse each) and profiles/gj_sweep_c4_mfma.txt (per kernel: MFMA busy cycles over
     GRBM_GUI_ACTIVE x 1024 SIMDs, and the MFMA flops the MOPS counters count)."""
-    out = out or os.path.join(ROOT, "gpurun_out", "prof_c4")
+    out = out or os.path.join(ROOT, "profile_out", "prof_c4")
     stats = list(csv.DictReader(open(os.path.join(out, "trace", "c4_kernel_stats.csv"))))
     bench = None
     with open(os.path.join(out, "trace", "..", "trace.log")) as f:

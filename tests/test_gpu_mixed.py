@@ -1,6 +1,6 @@
 """GPU checks of the mixed-precision solver (equilibrated FP32 LU + FP64 refinement + FP64 fallback).
 
-Tolerance (stated by the north-star contract): per-cell normwise relative error
+Tolerance (the stated mixed-precision bound): per-cell normwise relative error
 ||x - x_oracle||_inf / ||x_oracle||_inf <= 1e-8 after refinement, where the oracle is the FP64
 GSL-order solve.  Cells the solver hands back to FP64 must then equal the oracle exactly.
 """
