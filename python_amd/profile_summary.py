@@ -1,6 +1,8 @@
 """Summarise the rocprofv3 output of python_amd/profile_bench.sh into profiles/.
 
     python -m python_amd.profile_summary [OUT_DIR]      (default profile_out/rocprof)
+    python -m python_amd.profile_summary c4 [OUT_DIR]   (python_amd/profile_c4.sh output)
+    python -m python_amd.profile_summary stalls [OUT_DIR] (python_amd/profile_stalls.sh output)
 
 Reads the kernel statistics and kernel trace of `bench.py --roofline-only --no-cpu`, its
 FETCH_SIZE and WRITE_SIZE counter runs and the roofline object the command printed, and writes
@@ -274,8 +276,50 @@ def main_c4(out=None):
     print("\n".join(lines + [""] + ml))
 
 
+STALL_CMD = ("python_amd/profile_stalls.sh  (rocprofv3 --pmc <8 SQ counters>, twice, -- python3 bench.py "
+             "--roofline-only --no-cpu)")
+
+
+def main_stalls(out=None):
+    """profiles/stall_counters_uneven10k.txt: per kernel of the bench workload, the fraction of
+    wave cycles spent waiting (SQ_WAIT_ANY: on memory, LDS, barriers), waiting to issue
+    (SQ_WAIT_INST_ANY), issuing VALU and LDS instructions, and LDS bank conflicts per LDS cycle."""
+    out = out or os.path.join(ROOT, "profile_out", "stalls")
+    agg = {}
+    for sub in ("p1", "p2"):
+        with open(os.path.join(out, sub, "st_counter_collection.csv")) as f:
+            for r in csv.DictReader(f):
+                k = r["Kernel_Name"]
+                if not (k.startswith("void k_") or k.startswith("k_")):
+                    continue
+                k = k.replace("void ", "").split("(")[0]
+                agg.setdefault(k, {})
+                agg[k][r["Counter_Name"]] = agg[k].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    lines = ["# Stall counters of the bench workload's kernels (uneven 10k-cell batch, size classes in isolation)",
+             f"command: {STALL_CMD}", "",
+             "Fractions of SQ_WAVE_CYCLES (summed over the kernel's dispatches): wait = SQ_WAIT_ANY (a wave",
+             "waiting on memory, LDS or a barrier), issue-wait = SQ_WAIT_INST_ANY (ready, not issued), valu =",
+             "SQ_ACTIVE_INST_VALU, lds = SQ_ACTIVE_INST_LDS; bank = SQ_LDS_BANK_CONFLICT per LDS-active cycle.", "",
+             f"{'kernel':44s} {'wait':>6s} {'issue-wait':>10s} {'valu':>6s} {'lds':>6s} {'bank':>6s}"]
+    for k in sorted(agg, key=lambda k: -agg[k].get("SQ_WAVE_CYCLES", 0.0)):
+        d = agg[k]
+        wc = d.get("SQ_WAVE_CYCLES", 0.0)
+        if wc <= 0:
+            continue
+        lds = d.get("SQ_ACTIVE_INST_LDS", 0.0)
+        lines.append(f"{k[:44]:44s} {d.get('SQ_WAIT_ANY', 0.0) / wc:6.2f} {d.get('SQ_WAIT_INST_ANY', 0.0) / wc:10.2f} "
+                     f"{d.get('SQ_ACTIVE_INST_VALU', 0.0) / wc:6.2f} {lds / wc:6.2f} "
+                     f"{d.get('SQ_LDS_BANK_CONFLICT', 0.0) / lds if lds else 0.0:6.3f}")
+    os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
+    with open(os.path.join(ROOT, "profiles", "stall_counters_uneven10k.txt"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "c4":
         main_c4(sys.argv[2] if len(sys.argv) > 2 else None)
+    elif len(sys.argv) > 1 and sys.argv[1] == "stalls":
+        main_stalls(sys.argv[2] if len(sys.argv) > 2 else None)
     else:
         main(sys.argv[1] if len(sys.argv) > 1 else None)
