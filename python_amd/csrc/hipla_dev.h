@@ -36,11 +36,21 @@ struct SolveArgs
   // back-substitution kernel; with HIPLA_FLAG_KEEP_FACTORS, F is the kept LU store
   double *F;
   double *Y;
+  // a launch does its work only when its list length lies in [cnt_lo, cnt_hi] (otherwise every
+  // workgroup returns at once): two kernel shapes launched back to back on one device-side list
+  // length, each for the lengths it is faster at
+  int cnt_lo = 0;
+  int cnt_hi = 0x7fffffff;
 };
 
 __device__ __forceinline__ int list_len (const SolveArgs &s)
 {
   return s.dcount ? min (*s.dcount, s.nlist) : s.nlist;
+}
+
+__device__ __forceinline__ bool list_in_window (const SolveArgs &s, int len)
+{
+  return len >= s.cnt_lo && len <= s.cnt_hi;
 }
 
 // ----------------------------------------------------------------------------------------

@@ -25,6 +25,12 @@ hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, bool keep
 hipError_t launch_bsub_cells (const SolveArgs &a, int nmax, hipStream_t st);
 int blocked_f64_wgs_per_cu (int nmax);   // resident workgroups per CU (persistent grid size)
 hipError_t setup_solve_f64_attrs (size_t max_smem);
+// one-wave panel + look-ahead factorization, n <= 256 (k_lookahead.hip); back substitution by
+// k_bsub_f64 as for the blocked kernel
+size_t lookahead_f64_smem ();
+hipError_t launch_lookahead_f64 (const SolveArgs &a, int grid, bool keep, hipStream_t st);
+hipError_t setup_lookahead_attrs (size_t max_smem);
+int lookahead_f64_max_cells ();   // list lengths up to this run k_lookahead_f64 (n <= 256)
 int wgr_f64_limit ();   // largest n of the workgroup-resident FP64 route
 int tile_f64_limit ();
 hipError_t launch_tile_f64 (const SolveArgs &a, int nmax, bool keep, int grid, hipStream_t st);
@@ -34,8 +40,10 @@ int fp64_family_of (int nmax);
 int fp64_variants (int family, int nmax);
 hipError_t launch_fp64_variant (const SolveArgs &a, int nmax, int variant, bool keep, int grid, hipStream_t st,
                                 bool with_bsub = true);
-void fp64_variant_symbol (int nmax, int variant, bool keep, char *buf, size_t len);
-void fp64_kernel_symbol (int nmax, bool keep, char *buf, size_t len);
+// count: the group's cell count (-1: unknown), which decides between the look-ahead and the
+// blocked kernel for n <= 256
+void fp64_variant_symbol (int nmax, int variant, bool keep, char *buf, size_t len, int count = -1);
+void fp64_kernel_symbol (int nmax, bool keep, char *buf, size_t len, int count = -1);
 
 // ---- FP64 MFMA Gauss-Jordan inverse of row-diagonally dominant cells (k_gj.hip) ----
 struct GjArgs

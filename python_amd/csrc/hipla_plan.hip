@@ -327,7 +327,7 @@ extern "C" int hipla_plan_describe (hipla_plan p, char *buf, int buflen)
     // the FP64 kernel instantiation that runs the group (FP64 plans, or a mixed size class
     // routed straight to FP64)
     char sym[96];
-    fp64_variant_symbol (g.nmax, g.variant, (p->flags & HIPLA_FLAG_KEEP_FACTORS) != 0, sym, sizeof sym);
+    fp64_variant_symbol (g.nmax, g.variant, (p->flags & HIPLA_FLAG_KEEP_FACTORS) != 0, sym, sizeof sym, g.count);
     char line2[512];
     snprintf (line2, sizeof line2,
               "%s{\"kernel\": \"%s\", \"param\": %d, \"nmax\": %d, \"count\": %d, \"grid\": %d, "

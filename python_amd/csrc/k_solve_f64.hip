@@ -588,6 +588,8 @@ __global__ __launch_bounds__ (BS, BS == 256 ? 2 : 4) void k_blocked_f64 (SolveAr
   const int wv = __builtin_amdgcn_readfirstlane (tid >> 6);   // wave-uniform
 
   const int neff = list_len (s);
+  if (!list_in_window (s, neff))
+    return;
   for (int item = blockIdx.x; item < neff; item += gridDim.x)
   {
     const int cell = s.list[item];
@@ -1176,7 +1178,30 @@ hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, bool keep
     return hipErrorInvalidValue;
   const size_t sm = blocked_f64_smem (nmax);
   if (nmax <= 256)
-    launch_blocked_m<256, 16, true> (a, grid, sm, keep, st);
+  {
+    // up to one cell per CU the per-cell latency decides (one-wave panel with look-ahead); with
+    // more cells per CU, throughput (two four-wave workgroups per CU sharing the panel chains);
+    // a device-side list length gets both launches, each working only in its own range
+    const int lim = lookahead_f64_max_cells ();
+    if (a.dcount)
+    {
+      SolveArgs lo = a, hi = a;
+      lo.cnt_hi = lim;
+      hi.cnt_lo = lim + 1;
+      hipError_t e = launch_lookahead_f64 (lo, std::min (grid, lim), keep, st);
+      if (e != hipSuccess)
+        return e;
+      launch_blocked_m<256, 16, true> (hi, grid, sm, keep, st);
+    }
+    else if (a.nlist <= lim)
+    {
+      hipError_t e = launch_lookahead_f64 (a, std::min (grid, a.nlist), keep, st);
+      if (e != hipSuccess)
+        return e;
+    }
+    else
+      launch_blocked_m<256, 16, true> (a, grid, sm, keep, st);
+  }
   else if (nmax <= 512)
     launch_blocked_m<BLK_BS, 16, true> (a, grid, sm, keep, st);
   else
@@ -1189,9 +1214,10 @@ hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, bool keep
 
 // Name of the FP64 kernel instantiation a group of largest order nmax runs (as profilers print
 // it), for the plan description.
-void fp64_kernel_symbol (int nmax, bool keep, char *buf, size_t len)
+void fp64_kernel_symbol (int nmax, bool keep, char *buf, size_t len, int count)
 {
   const char *k = keep ? "true" : "false";
+  const bool la = count >= 0 && count <= lookahead_f64_max_cells ();
   if (nmax <= 64)
   {
     const int nc = nmax <= 16 ? 16 : nmax <= 32 ? 32 : 64;
@@ -1203,6 +1229,8 @@ void fp64_kernel_symbol (int nmax, bool keep, char *buf, size_t len)
     tile_kernel_symbol (nmax, keep, t, sizeof t);
     snprintf (buf, len, "%s + k_bsub_f64<%d>", t, nmax <= 128 ? 2 : 3);
   }
+  else if (nmax <= 256 && la)
+    snprintf (buf, len, "k_lookahead_f64<%s> + k_bsub_f64<%d>", k, nmax <= 128 ? 2 : nmax <= 192 ? 3 : 4);
   else if (nmax <= 512)
     snprintf (buf, len, "k_blocked_f64<%d, 16, true, %s> + k_bsub_f64<%d>", nmax <= 256 ? 256 : BLK_BS, k,
               nmax <= 128 ? 2 : nmax <= 192 ? 3 : nmax <= 256 ? 4 : 8);
@@ -1242,7 +1270,7 @@ hipError_t setup_solve_f64_attrs (size_t max_smem)
   set_smem_attr (k_bsub_f64<4>, max_smem);
   set_smem_attr (k_bsub_f64<8>, max_smem);
   set_smem_attr (k_bsub_f64<BLK_MAXM>, max_smem);
-  return hipSuccess;
+  return setup_lookahead_attrs (max_smem);
 }
 
 // Kernel shapes of one FP64 kernel family (equal values: every variant keeps GSL's order), for
@@ -1300,18 +1328,20 @@ hipError_t launch_fp64_variant (const SolveArgs &a, int nmax, int variant, bool 
 }
 
 // Name of the kernel instantiation(s) a variant runs (as profilers print it)
-void fp64_variant_symbol (int nmax, int variant, bool keep, char *buf, size_t len)
+void fp64_variant_symbol (int nmax, int variant, bool keep, char *buf, size_t len, int count)
 {
   const int fam = fp64_family_of (nmax);
   if (variant <= 0 || variant >= fp64_variants (fam, nmax))
   {
-    fp64_kernel_symbol (nmax, keep, buf, len);
+    fp64_kernel_symbol (nmax, keep, buf, len, count);
     return;
   }
   const char *k = keep ? "true" : "false";
   const int m = nmax <= 128 ? 2 : nmax <= 192 ? 3 : nmax <= 256 ? 4 : 8;
   if (fam < 3)
     snprintf (buf, len, "k_wave_f64<%d, %d, %s>", fam == 0 ? 16 : fam == 1 ? 32 : 64, fam == 2 ? 1 : 2, k);
+  else if (fam == 3 && count >= 0 && count <= lookahead_f64_max_cells ())
+    snprintf (buf, len, "k_lookahead_f64<%s> + k_bsub_f64<%d>", k, m);
   else if (fam == 3)
     snprintf (buf, len, "k_blocked_f64<256, 16, true, %s> + k_bsub_f64<%d>", k, m);
   else
