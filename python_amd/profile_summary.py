@@ -3,6 +3,7 @@
     python -m python_amd.profile_summary [OUT_DIR]      (default profile_out/rocprof)
     python -m python_amd.profile_summary c4 [OUT_DIR]   (python_amd/profile_c4.sh output)
     python -m python_amd.profile_summary stalls [OUT_DIR] (python_amd/profile_stalls.sh output)
+    python -m python_amd.profile_summary ion [OUT_DIR]  (python_amd/profile_ion.sh output)
 
 Reads the kernel statistics and kernel trace of `bench.py --roofline-only --no-cpu`, its
 FETCH_SIZE and WRITE_SIZE counter runs and the roofline object the command printed, and writes
@@ -315,11 +316,74 @@ def main_stalls(out=None):
         f.write("\n".join(lines) + "\n")
     print("\n".join(lines))
 
+ION_CMD = ("python_amd/profile_ion.sh  (rocprofv3 --kernel-trace --stats --output-format csv -- python3 bench.py "
+           "--config ion --steps 1 --warmup 1 --no-cpu)")
+
+
+def main_ion(out=None):
+    """profiles/ion_iteration_uneven10k_trace.txt: the last populations() call of the ion bench
+    (every cell's whole n_e iteration): per-kernel dispatch counts and times, and the wall time of
+    an iteration (assembly start to update end) in the first iterations, with most cells active,
+    and in the tail, where a few cells are left and the order 175 < n <= 256 route decides."""
+    out = out or os.path.join(ROOT, "profile_out", "ion")
+    path = None
+    for dp, _, fs in os.walk(os.path.join(out, "trace")):
+        for f in fs:
+            if f.endswith("kernel_trace.csv"):
+                path = os.path.join(dp, f)
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"].replace("void ", "").split("(")[0]))
+    rows.sort()
+    last_init = max(i for i, r in enumerate(rows) if r[2] == "k_ion_init")
+    call = [r for r in rows[last_init:] if r[2].startswith("k_")]
+    t_call = (max(r[1] for r in call) - call[0][0]) / 1e6
+    stats = {}
+    for s0, e0, k in call:
+        d = stats.setdefault(k, [0, 0.0])
+        d[0] += 1
+        d[1] += (e0 - s0) / 1e6
+    # iterations: from each assembly to the update that follows it
+    its = []
+    cur = None
+    for s0, e0, k in call:
+        if k == "k_ion_assemble":
+            cur = {"start": s0, "big": 0.0}
+        elif cur is not None and k.startswith(("k_lookahead_f64", "k_blocked_f64<256", "k_bsub_f64<4>")):
+            cur["big"] += (e0 - s0) / 1e6
+        elif cur is not None and k == "k_ion_update":
+            cur["end"] = e0
+            its.append(cur)
+            cur = None
+    wall = [(i["end"] - i["start"]) / 1e6 for i in its]
+    big = [i["big"] for i in its]
+    lines = ["# Lock-step ionization iteration (10k uneven cells, n in {12..256}, seed 6): one populations() call",
+             f"command: {ION_CMD}", "",
+             f"call: {t_call:.2f} ms over {len(its)} iterations (assembly start to update end of each, graph",
+             "replays of four iterations with a host read of the active count between them)", "",
+             f"{'kernel':34s} {'calls':>6s} {'total ms':>9s} {'avg us':>8s}"]
+    for k in sorted(stats, key=lambda k: -stats[k][1]):
+        c, t = stats[k]
+        lines.append(f"{k[:34]:34s} {c:6d} {t:9.2f} {1e3 * t / c:8.1f}")
+    def band(lo, hi):
+        w = wall[lo:hi]
+        b = big[lo:hi]
+        return (f"iterations {lo + 1}..{min(hi, len(wall))}: wall {sum(w) / len(w):.3f} ms per iteration, of which the "
+                f"175 < n <= 256 route (factor kernels + k_bsub_f64<4>) {sum(b) / len(b):.3f} ms") if w else ""
+    lines += ["", band(0, 6), band(6, 40), band(40, len(wall))]
+    os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
+    with open(os.path.join(ROOT, "profiles", "ion_iteration_uneven10k_trace.txt"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
 
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "c4":
         main_c4(sys.argv[2] if len(sys.argv) > 2 else None)
     elif len(sys.argv) > 1 and sys.argv[1] == "stalls":
         main_stalls(sys.argv[2] if len(sys.argv) > 2 else None)
+    elif len(sys.argv) > 1 and sys.argv[1] == "ion":
+        main_ion(sys.argv[2] if len(sys.argv) > 2 else None)
     else:
         main(sys.argv[1] if len(sys.argv) > 1 else None)
