@@ -190,8 +190,8 @@ __device__ __forceinline__ const double *la_src_row (const double *A, const doub
   return src + (size_t) q * n;
 }
 
-// acc[m][0 .. H-1] -= l[m] * u[0 .. H-1] for the rows m < mr (uniform): each row's products in
-// registers of their own, then its subtractions.
+// acc[m][0 .. H-1] -= l[m] * u[0 .. H-1] for the row slots m that hold rows (64 m < rows, a
+// wave-uniform test): each row's products in registers of their own, then its subtractions.
 template <int H>
 __device__ __forceinline__ void la_rank1 (double (&acc)[LA_MR][H], const double (&l)[LA_MR], const double (&u)[H],
                                           int rows)
@@ -241,7 +241,6 @@ __device__ __forceinline__ void la_trail_strip (const double *A, double *W, cons
     u[kk][3] = b.y;
   }
   const int nt = (rows1 + 16 * LA_TR - 1) / (16 * LA_TR);
-  double nx[LA_TR][4];
   auto load_tile = [&] (int t, double (&d)[LA_TR][4]) {
 #pragma unroll
     for (int r = 0; r < LA_TR; r++)
