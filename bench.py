@@ -178,7 +178,7 @@ def bench_c1(args):
                                   "ms_per_step": round(ms_per, 3)},
         "roofline": {"bound": "hbm", "achieved": round(bw, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(bw / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "k_wave_f64<32, 4, false> (one graph launch)",
+                     "kernel": plan.describe()["groups"][0]["fp64_kernel"] + " (one graph launch)",
                      "algorithmic_bytes_per_launch": round(byts), "algorithmic_flops_per_launch": round(flops)},
         "cpu_baseline": {"value": round(reps_c * batch.ncells / total, 1), "unit": "solves/s", "cores": cores,
                          "kind": "port", "sample": f"the 900-cell batch x {reps_c} ({total:.1f} s, one OpenMP "

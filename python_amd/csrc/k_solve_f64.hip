@@ -1112,7 +1112,7 @@ hipError_t launch_wave_f64 (const SolveArgs &a, int nc, bool keep, int grid, hip
   case 16:
     return launch_wave_nc<16, 4> (a, keep, grid, st);
   case 32:
-    return launch_wave_nc<32, 4> (a, keep, grid, st);
+    return launch_wave_nc<32, 2> (a, keep, grid, st);
   case 64:
     return launch_wave_nc<64, 2> (a, keep, grid, st);
   default:
@@ -1221,7 +1221,7 @@ void fp64_kernel_symbol (int nmax, bool keep, char *buf, size_t len, int count)
   if (nmax <= 64)
   {
     const int nc = nmax <= 16 ? 16 : nmax <= 32 ? 32 : 64;
-    snprintf (buf, len, "k_wave_f64<%d, %d, %s>", nc, nc == 64 ? 2 : 4, k);
+    snprintf (buf, len, "k_wave_f64<%d, %d, %s>", nc, nc == 16 ? 4 : 2, k);
   }
   else if (nmax <= wgr_f64_limit ())
   {
@@ -1310,7 +1310,7 @@ hipError_t launch_fp64_variant (const SolveArgs &a, int nmax, int variant, bool 
   case 0:
     return launch_wave_nc<16, 2> (a, keep, grid, st);
   case 1:
-    return launch_wave_nc<32, 2> (a, keep, grid, st);
+    return launch_wave_nc<32, 4> (a, keep, grid, st);
   case 2:
     return launch_wave_nc<64, 1> (a, keep, grid, st);
   case 3:
@@ -1339,7 +1339,7 @@ void fp64_variant_symbol (int nmax, int variant, bool keep, char *buf, size_t le
   const char *k = keep ? "true" : "false";
   const int m = nmax <= 128 ? 2 : nmax <= 192 ? 3 : nmax <= 256 ? 4 : 8;
   if (fam < 3)
-    snprintf (buf, len, "k_wave_f64<%d, %d, %s>", fam == 0 ? 16 : fam == 1 ? 32 : 64, fam == 2 ? 1 : 2, k);
+    snprintf (buf, len, "k_wave_f64<%d, %d, %s>", fam == 0 ? 16 : fam == 1 ? 32 : 64, fam == 0 ? 2 : fam == 1 ? 4 : 1, k);
   else if (fam == 3 && count >= 0 && count <= lookahead_f64_max_cells ())
     snprintf (buf, len, "k_lookahead_f64<%s> + k_bsub_f64<%d>", k, m);
   else if (fam == 3)
