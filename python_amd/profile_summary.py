@@ -4,6 +4,7 @@
     python -m python_amd.profile_summary c4 [OUT_DIR]   (python_amd/profile_c4.sh output)
     python -m python_amd.profile_summary stalls [OUT_DIR] (python_amd/profile_stalls.sh output)
     python -m python_amd.profile_summary ion [OUT_DIR]  (python_amd/profile_ion.sh output)
+    python -m python_amd.profile_summary timeline [OUT_DIR] (python_amd/profile_timeline.sh output)
 
 Reads the kernel statistics and kernel trace of `bench.py --roofline-only --no-cpu`, its
 FETCH_SIZE and WRITE_SIZE counter runs and the roofline object the command printed, and writes
@@ -378,6 +379,74 @@ def main_ion(out=None):
     print("\n".join(lines))
 
 
+TL_CMD = ("python_amd/profile_timeline.sh  (rocprofv3 --kernel-trace --output-format csv -- python3 -m "
+          "python_amd.profile_summary run_timeline)")
+
+
+def run_timeline():
+    """The traced workload: the bench's headline plan (uneven 10k-cell batch, mixed mode, autotuned,
+    graph replays), three warm-up solves, a spin kernel as a marker, then two solves."""
+    import torch
+    import python_amd as hip
+    from python_amd.generators import uneven
+    b = uneven()
+    A, B = hip.to_device(b)
+    X = torch.zeros_like(B)
+    codes = torch.empty(b.ncells, dtype=torch.int32, device=B.device)
+    plan = hip.Plan.for_batch(b, mode="mixed", graph=True)
+    plan.autotune(A, B, X, codes)
+    for _ in range(3):
+        plan.solve(A, B, X, codes)
+    torch.cuda.synchronize()
+    torch.cuda._sleep(1000000)
+    torch.cuda.synchronize()
+    for _ in range(2):
+        plan.solve(A, B, X, codes)
+    torch.cuda.synchronize()
+    print("groups", [(g["nmax"], g["count"], g["fp64_kernel"]) for g in plan.describe()["groups"]], flush=True)
+
+
+def main_timeline(out=None):
+    """profiles/c2_step_timeline_uneven10k.txt: the kernels of the last traced solve of the headline
+    plan, with their queue, start and end (us from the step's first kernel), grid, registers and
+    static LDS, as the profiler saw them."""
+    out = out or os.path.join(ROOT, "profile_out", "timeline")
+    path = None
+    for dp, _, fs in os.walk(os.path.join(out, "trace")):
+        for f in fs:
+            if f.endswith("kernel_trace.csv"):
+                path = os.path.join(dp, f)
+    with open(path) as f:
+        rows = sorted(csv.DictReader(f), key=lambda r: int(r["Start_Timestamp"]))
+    mark = max(i for i, r in enumerate(rows) if "spin" in r["Kernel_Name"] or "sleep" in r["Kernel_Name"])
+    after = rows[mark + 1:]
+    # the second marked solve starts with the second launch of the step's first kernel
+    first = after[0]["Kernel_Name"]
+    starts = [i for i, r in enumerate(after) if r["Kernel_Name"] == first]
+    step = after[starts[1]:] if len(starts) > 1 else after
+    t0 = int(step[0]["Start_Timestamp"])
+    t_end = max(int(r["End_Timestamp"]) for r in step)
+    lines = ["# One solve of the headline plan (uneven 10k-cell batch, mixed mode, autotuned: every class in FP64)",
+             f"command: {TL_CMD}", "",
+             f"step: {(t_end - t0) / 1e3:.1f} us from the first kernel's start to the last kernel's end "
+             "(profiled; queue = HIP stream)", "",
+             "VGPR as the trace reports it (units of two registers); LDS the static part only (the factor",
+             "kernels' dynamic LDS: k_blocked_f64<256> 72 KB per workgroup at n = 256, k_wave_f64<64> 9.4 KB,",
+             "k_wave_f64<32, 2> 9.6 KB, k_wave_f64<16, 4> 11 KB).", "",
+             f"{'start':>8s} {'end':>8s} {'dur':>7s} {'queue':>5s} {'grid':>6s} {'vgpr':>5s} {'lds':>6s}  kernel"]
+    for r in step:
+        s0 = (int(r["Start_Timestamp"]) - t0) / 1e3
+        e0 = (int(r["End_Timestamp"]) - t0) / 1e3
+        grid = int(r["Grid_Size_X"]) // max(1, int(r["Workgroup_Size_X"]))
+        name = r["Kernel_Name"].replace("void ", "").split("(")[0]
+        lines.append(f"{s0:8.1f} {e0:8.1f} {e0 - s0:7.1f} {r['Queue_Id']:>5s} {grid:6d} {r['VGPR_Count']:>5s} "
+                     f"{r['LDS_Block_Size']:>6s}  {name}")
+    os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
+    with open(os.path.join(ROOT, "profiles", "c2_step_timeline_uneven10k.txt"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "c4":
         main_c4(sys.argv[2] if len(sys.argv) > 2 else None)
@@ -385,5 +454,9 @@ if __name__ == "__main__":
         main_stalls(sys.argv[2] if len(sys.argv) > 2 else None)
     elif len(sys.argv) > 1 and sys.argv[1] == "ion":
         main_ion(sys.argv[2] if len(sys.argv) > 2 else None)
+    elif len(sys.argv) > 1 and sys.argv[1] == "run_timeline":
+        run_timeline()
+    elif len(sys.argv) > 1 and sys.argv[1] == "timeline":
+        main_timeline(sys.argv[2] if len(sys.argv) > 2 else None)
     else:
         main(sys.argv[1] if len(sys.argv) > 1 else None)
