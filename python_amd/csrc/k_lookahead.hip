@@ -12,27 +12,30 @@
 //  * A panel is factored by ONE wave: lane l holds the panel rows l, l+64, l+128, l+192 (16
 //    columns and b) in registers.  A pivot step is a choice among the lane's four rows, a wave
 //    reduction and readlanes of the pivot row into scalar registers: no barrier.
-//  * Look-ahead: in the epoch that applies panel p, wave (p+1) mod 4 updates the columns of panel
-//    p+1 with panel p and factors panel p+1 at once, while the other three waves apply panel p to
+//  * Look-ahead: while wave (p+1) mod 4 factors panel p+1, the other three waves apply panel p to
 //    the columns right of it; the panel chain overlaps that trailing update.
+//  * The kernel runs for up to one cell per CU, so a workgroup has the CU to itself: 140 KB of LDS
+//    hold two panel buffers (the panel being applied and the one being factored), the U12 rows
+//    and the next panel's updated rows.
 //  * The trailing update runs in units of one 16-column strip x all remaining rows: a lane keeps
 //    its four columns' U12 values in registers and reads one multiplier pair per row and two
 //    steps from LDS; units are handed out from an LDS counter.
 //
-// Epoch p (panel columns k0 .. k1-1) is three stages, one barrier apart:
-//   A. the panel's rows (factored and written in position order to the factor store by the wave
-//      that factored it, at the end of the previous epoch) are copied to LDS (P);
+// Epoch p (panel columns k0 .. k1-1; panel p already factored, in LDS and in the factor store in
+// position order) is three stages, one barrier apart:
 //   U. one thread per column outside the panel moves the rows the panel's pivoting moved (the
 //      panel rows, and at most 16 rows displaced below it) and, right of the panel, forms
 //      U12 = L11^-1 A12 in ascending order, into the factor store and LDS (Us);
-//   T. wave (p+1) mod 4 loads the columns of panel p+1 (rows k1 ..), applies panel p, factors
-//      panel p+1 and writes it, b, its position map and GSL's ipiv out; the other waves apply
-//      A22 -= L21 U12 to the columns from k1 + 16 on.
+//   V. one thread per row applies panel p's 16 updates to the columns of panel p+1 and puts the
+//      row, with its b, in LDS (Sn);
+//   T. wave (p+1) mod 4 factors panel p+1 from Sn and writes it out (the other panel buffer, the
+//      factor store, b, its position map, GSL's ipiv); the other waves apply A22 -= L21 U12 to
+//      the columns from k1 + 16 on.
 // The first panel reads A and b directly; a row of A that no pivoting moved is read from A until
 // the first trailing update has written it to the factor store.
 //
 // Against k_blocked_f64<256> (four-wave panel steps about 2 400 cycles apart, two workgroups per
-// CU filling each other's gaps), this kernel is faster for one cell (0.74 against 0.86 ms at
+// CU filling each other's gaps), this kernel is faster for one cell (0.69 against 0.86 ms at
 // n = 256, back substitution included) and slower when several cells share each CU beside other
 // size classes, so it runs for up to one cell per CU.
 #include <hip/hip_runtime.h>
@@ -306,15 +309,19 @@ __device__ __forceinline__ void la_trail_strip (const double *A, double *W, cons
 }
 
 template <bool KEEP>
-__global__ __launch_bounds__ (256, 2) void k_lookahead_f64 (SolveArgs s)
+__global__ __launch_bounds__ (256, 1) void k_lookahead_f64 (SolveArgs s)
 {
   extern __shared__ double smem[];
-  double *P = smem;                          // [LA_MAXN][LA_PP] panel rows by relative position
-  double *Us = smem + LA_MAXN * LA_PP;       // [LA_NB][LA_UP] U12, columns relative to k1
+  // two panel buffers by parity ([LA_MAXN][LA_PP]: multipliers and U11 by relative position), the
+  // U12 rows ([LA_NB][LA_UP], columns relative to k1) and the next panel's rows ([LA_MAXN][LA_PP]:
+  // its 16 columns and b, updated, by relative position)
+  double *Pbuf = smem;
+  double *Us = smem + 2 * LA_MAXN * LA_PP;
+  double *Sn = Us + LA_NB * LA_UP;
   __shared__ int phys[2][LA_MAXN];           // per panel parity: relative position -> row it held
-  __shared__ int mv_dst[LA_NB], mv_src[LA_NB];
+  __shared__ int mv_dst[2][LA_NB], mv_src[2][LA_NB], s_nmv[2];
   __shared__ int s_ipiv[LA_NB];
-  __shared__ int s_nmv, s_next, s_sing;
+  __shared__ int s_next, s_sing;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = la_uni (tid >> 6);
   const int neff = list_len (s);
@@ -346,37 +353,25 @@ __global__ __launch_bounds__ (256, 2) void k_lookahead_f64 (SolveArgs s)
       const int k0 = p * LA_NB;
       const int nbk = min (LA_NB, n - k0);
       const int k1 = k0 + nbk;                  // p = -1: 0
-      const int *ph = phys[p & 1];
+      const int rows1 = n - k1;
+      const int pb = p & 1;
+      const double *P = Pbuf + pb * LA_MAXN * LA_PP;
+      const int *ph = phys[pb];
+      const bool first = p == 0;
       if (p >= 0)
       {
-        const int rows = n - k0;
         __syncthreads ();
-
-        // ---- A. panel p (factored and written in position order by its wave) into LDS ----
-        if (tid < rows)
-        {
-          const double *src = W + (size_t) (k0 + tid) * n + k0;
-          double t[LA_NB];
-#pragma unroll
-          for (int c = 0; c < LA_NB; c++)
-            t[c] = c < nbk ? src[min (c, nbk - 1)] : 0.0;
-          double *dst = P + tid * LA_PP;
-#pragma unroll
-          for (int c = 0; c < LA_NB; c += 2)
-            *reinterpret_cast<double2 *> (dst + c) = make_double2 (t[c], t[c + 1]);
-        }
+        const int nmv = la_uni (s_nmv[pb]);
         if (tid == 0)
           s_next = 0;
-        __syncthreads ();
 
         // ---- U. row moves outside the panel, U12 right of it ----
         {
           const int c = tid;
           if (c < n && (c < k0 || c >= k1))
           {
-            const double *S = p == 0 ? A : W;
+            const double *S = first ? A : W;
             const int last = nbk - 1;
-            const int nmv = la_uni (s_nmv);
             double u[LA_NB];
 #pragma unroll
             for (int kk = 0; kk < LA_NB; kk++)
@@ -386,11 +381,11 @@ __global__ __launch_bounds__ (256, 2) void k_lookahead_f64 (SolveArgs s)
               double dv[LA_NB];
 #pragma unroll
               for (int i = 0; i < LA_NB; i++)
-                dv[i] = S[(size_t) (k0 + la_uni (mv_src[min (i, nmv - 1)])) * n + c];
+                dv[i] = S[(size_t) (k0 + la_uni (mv_src[pb][min (i, nmv - 1)])) * n + c];
 #pragma unroll
               for (int i = 0; i < LA_NB; i++)
                 if (i < nmv)
-                  W[(size_t) (k0 + la_uni (mv_dst[i])) * n + c] = dv[i];
+                  W[(size_t) (k0 + la_uni (mv_dst[pb][i])) * n + c] = dv[i];
             }
             if (c >= k1)
             {
@@ -420,76 +415,101 @@ __global__ __launch_bounds__ (256, 2) void k_lookahead_f64 (SolveArgs s)
           }
         }
         __syncthreads ();
+
+        // ---- V. the next panel's columns minus panel p's products, one row per thread, into Sn ----
+        if (p + 1 < ns && tid < rows1)
+        {
+          const int nbk1 = min (LA_NB, rows1);
+          const double *row = la_src_row (A, W, ph, first, k1 + tid, n) + k1;
+          double a[LA_NB];
+#pragma unroll
+          for (int c = 0; c < LA_NB; c++)
+            a[c] = c < nbk1 ? row[min (c, nbk1 - 1)] : 0.0;
+          const double *Lr = P + (LA_NB + tid) * LA_PP;
+          // panel p's 16 updates in ascending order (panel p is full: a column follows it)
+#pragma unroll
+          for (int kk = 0; kk < LA_NB; kk += 2)
+          {
+            const double2 l2 = *reinterpret_cast<const double2 *> (Lr + kk);
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+            {
+              const double l = h ? l2.y : l2.x;
+              double u[LA_NB];
+#pragma unroll
+              for (int c = 0; c < LA_NB; c += 2)
+              {
+                const double2 t = *reinterpret_cast<const double2 *> (Us + (kk + h) * LA_UP + c);
+                u[c] = t.x;
+                u[c + 1] = t.y;
+              }
+              double q[LA_NB];
+#pragma unroll
+              for (int c = 0; c < LA_NB; c++)
+              {
+                q[c] = l * u[c];
+                la_hold (q[c]);
+              }
+#pragma unroll
+              for (int c = 0; c < LA_NB; c++)
+                a[c] = a[c] - q[c];
+            }
+          }
+          double *dst = Sn + tid * LA_PP;
+#pragma unroll
+          for (int c = 0; c < LA_NB; c += 2)
+            *reinterpret_cast<double2 *> (dst + c) =
+              make_double2 (c < nbk1 ? a[c] : 0.0, c + 1 < nbk1 ? a[c + 1] : 0.0);
+          dst[LA_NB] = wb[k1 + tid];
+        }
+        __syncthreads ();
       }
 
       // ---- T. panel p+1 on one wave, A22 on the others ----
       if (p + 1 < ns)
       {
-        const int rows1 = n - k1;
-        const bool first = p == 0;
         if (wv == ((p + 1) & 3))
         {
           LaPanel pn;
           int singular = 0;
           const int nbk1 = min (LA_NB, rows1);
-          int pr[LA_MR];
 #pragma unroll
           for (int m = 0; m < LA_MR; m++)
           {
             const int r = lane + 64 * m;
             pn.mp[m] = r < rows1 ? r : LA_NONE;
             const int rc = min (r, rows1 - 1);
-            pr[m] = (LA_NB + rc) * LA_PP;
-            const double *row = (p < 0 ? A + (size_t) rc * n : la_src_row (A, W, ph, first, k1 + rc, n)) + k1;
-#pragma unroll
-            for (int c = 0; c < LA_NB; c++)
-              pn.v[m][c] = c < nbk1 ? row[min (c, nbk1 - 1)] : 0.0;
-            pn.v[m][LA_NB] = p < 0 ? b[rc] : wb[k1 + rc];
-          }
-          if (p >= 0)
-          {
-            // panel p's 16 updates, in ascending order (panel p is full: a column follows it)
-#pragma unroll
-            for (int kk = 0; kk < LA_NB; kk++)
+            if (p < 0)
             {
-              double l[LA_MR];
+              const double *row = A + (size_t) rc * n;
 #pragma unroll
-              for (int m = 0; m < LA_MR; m++)
-                l[m] = 64 * m < rows1 ? P[pr[m] + kk] : 0.0;
+              for (int c = 0; c < LA_NB; c++)
+                pn.v[m][c] = c < nbk1 ? row[min (c, nbk1 - 1)] : 0.0;
+              pn.v[m][LA_NB] = b[rc];
+            }
+            else
+            {
+              const double *src = Sn + rc * LA_PP;
 #pragma unroll
-              for (int h = 0; h < 2; h++)
+              for (int c = 0; c < LA_NB; c += 2)
               {
-                double u[LA_NB / 2];
-#pragma unroll
-                for (int e = 0; e < LA_NB / 2; e += 2)
-                {
-                  const double2 t = *reinterpret_cast<const double2 *> (Us + kk * LA_UP + 8 * h + e);
-                  u[e] = t.x;
-                  u[e + 1] = t.y;
-                }
-                double a[LA_MR][LA_NB / 2];
-#pragma unroll
-                for (int m = 0; m < LA_MR; m++)
-#pragma unroll
-                  for (int e = 0; e < LA_NB / 2; e++)
-                    a[m][e] = pn.v[m][8 * h + e];
-                la_rank1<LA_NB / 2> (a, l, u, rows1);
-#pragma unroll
-                for (int m = 0; m < LA_MR; m++)
-#pragma unroll
-                  for (int e = 0; e < LA_NB / 2; e++)
-                    pn.v[m][8 * h + e] = a[m][e];
+                const double2 t = *reinterpret_cast<const double2 *> (src + c);
+                pn.v[m][c] = t.x;
+                pn.v[m][c + 1] = t.y;
               }
+              pn.v[m][LA_NB] = src[LA_NB];
             }
           }
           __builtin_amdgcn_s_setprio (2);
           la_factor_from<0> (pn, nbk1, lane, s_ipiv, singular);
           __builtin_amdgcn_s_setprio (0);
 
-          // publish panel p+1 in position order: its columns to the factor store, b to Y, the
-          // position map (the other parity: the trailing waves may still read this epoch's),
-          // the rows pivoting moved below the panel, GSL's ipiv
-          int *phn = phys[(p + 1) & 1];
+          // publish panel p+1 in position order: multipliers and U11 to the other panel buffer, its
+          // columns to the factor store, b to Y, the position map, the rows pivoting moved below the
+          // panel, GSL's ipiv
+          const int nb1 = (p + 1) & 1;
+          double *Pn = Pbuf + nb1 * LA_MAXN * LA_PP;
+          int *phn = phys[nb1];
 #pragma unroll
           for (int m = 0; m < LA_MR; m++)
           {
@@ -497,6 +517,10 @@ __global__ __launch_bounds__ (256, 2) void k_lookahead_f64 (SolveArgs s)
             if (r < rows1)
             {
               const int q = pn.mp[m];
+              double *pr = Pn + q * LA_PP;
+#pragma unroll
+              for (int c = 0; c < LA_NB; c += 2)
+                *reinterpret_cast<double2 *> (pr + c) = make_double2 (pn.v[m][c], pn.v[m][c + 1]);
               double *wr = W + (size_t) (k1 + q) * n + k1;
 #pragma unroll
               for (int c = 0; c < LA_NB; c++)
@@ -512,12 +536,12 @@ __global__ __launch_bounds__ (256, 2) void k_lookahead_f64 (SolveArgs s)
           {
             const int i = __builtin_amdgcn_mbcnt_hi ((unsigned) (mask >> 32),
                                                      __builtin_amdgcn_mbcnt_lo ((unsigned) mask, 0u));
-            mv_dst[i] = pn.mp[0];
-            mv_src[i] = lane;
+            mv_dst[nb1][i] = pn.mp[0];
+            mv_src[nb1][i] = lane;
           }
           if (lane == 0)
           {
-            s_nmv = __popcll (mask);
+            s_nmv[nb1] = __popcll (mask);
             if (singular)
               s_sing = 1;
           }
@@ -556,7 +580,7 @@ int lookahead_f64_max_cells ()
 
 size_t lookahead_f64_smem ()
 {
-  return (size_t) (LA_MAXN * LA_PP + LA_NB * LA_UP) * sizeof (double);
+  return (size_t) (3 * LA_MAXN * LA_PP + LA_NB * LA_UP) * sizeof (double);
 }
 
 hipError_t launch_lookahead_f64 (const SolveArgs &a, int grid, bool keep, hipStream_t st)
