@@ -86,3 +86,56 @@ def test_ion_iteration_crosses_the_cell_limit(hip, oracle_lib):
     ok = wst == 0
     assert np.array_equal(ne[ok], wne[ok])
     assert np.array_equal(dens.view(np.uint64), wd.view(np.uint64))
+
+
+def _hard_cells(n, seed):
+    """A singular matrix (a zero column: code 4), near-singular ones (singular values down to 1e-17:
+    the residual check fires with codes 2 or 3 depending on the sign of b), and a regular one."""
+    rng = np.random.default_rng(seed)
+    mats, rhs = [], []
+    A = rng.standard_normal((n, n))
+    A[:, n // 3] = 0.0
+    mats.append(A)
+    rhs.append(rng.standard_normal(n))
+    for s in range(4):
+        U, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        A = (U * np.logspace(0, -17, n)) @ U.T
+        b = rng.standard_normal(n)
+        mats.append(A)
+        rhs.append(np.abs(b) if s % 2 else b)
+    A = rng.standard_normal((n, n))
+    A[np.arange(n), np.arange(n)] += 2 * np.sqrt(n)
+    mats.append(A)
+    rhs.append(rng.standard_normal(n))
+    return mats, rhs
+
+
+def test_singular_and_residual_codes_both_kernels(hip, oracle_lib):
+    """Codes 4 (x left untouched), 2 and 3 of singular and near-singular cells of order 200, in a
+    group small enough for the look-ahead kernel and in one large enough for the blocked kernel
+    (the same cells repeated), equal to the oracle's."""
+    import torch
+    from python_amd.generators import pack
+    mats, rhs = _hard_cells(200, 71)
+    ncu = torch.cuda.get_device_properties(0).multi_processor_count
+    reps = ncu // len(mats) + 2
+    for batch, want in ((pack(mats, rhs), "k_lookahead_f64"), (pack(mats * reps, rhs * reps), "k_blocked_f64<256")):
+        A, B = hip.to_device(batch)
+        plan = hip.Plan.for_batch(batch, mode="fp64", graph=False)
+        names = [g["fp64_kernel"] for g in plan.describe()["groups"]]
+        assert len(names) == 1 and names[0].startswith(want), names
+        X = torch.full_like(B, 7.0)
+        Xr, codes, _ = plan.solve(A, B, X)
+        torch.cuda.synchronize()
+        Xh, ch = Xr.cpu().numpy(), codes.cpu().numpy()
+        Xo, co = oracle_lib.solve_batch(batch.n, batch.offA, batch.offV, batch.A, batch.B, nthreads=8)
+        print(batch.ncells, want, "codes", np.bincount(co, minlength=5).tolist(), flush=True)
+        assert np.array_equal(ch, co)
+        assert {0, 4} <= set(co.tolist()) and (2 in co or 3 in co)
+        for c in range(batch.ncells):
+            o, n = batch.offV[c], batch.n[c]
+            if co[c] == 4:
+                assert np.all(Xh[o:o + n] == 7.0), c
+            else:
+                assert np.array_equal(Xh[o:o + n], Xo[o:o + n]), c
+        plan.close()
