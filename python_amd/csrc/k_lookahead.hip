@@ -26,8 +26,8 @@
 //   U. one thread per column outside the panel moves the rows the panel's pivoting moved (the
 //      panel rows, and at most 16 rows displaced below it) and, right of the panel, forms
 //      U12 = L11^-1 A12 in ascending order, into the factor store and LDS (Us);
-//   V. one thread per row applies panel p's 16 updates to the columns of panel p+1 and puts the
-//      row, with its b, in LDS (Sn);
+//   V. four lanes per row (U12 of their four columns in registers) apply panel p's 16 updates to
+//      the columns of panel p+1 and put the rows, with b, in LDS (Sn);
 //   T. wave (p+1) mod 4 factors panel p+1 from Sn and writes it out (the other panel buffer, the
 //      factor store, b, its position map, GSL's ipiv); the other waves apply A22 -= L21 U12 to
 //      the columns from k1 + 16 on.
@@ -35,7 +35,7 @@
 // the first trailing update has written it to the factor store.
 //
 // Against k_blocked_f64<256> (four-wave panel steps about 2 400 cycles apart, two workgroups per
-// CU filling each other's gaps), this kernel is faster for one cell (0.69 against 0.86 ms at
+// CU filling each other's gaps), this kernel is faster for one cell (0.67 against 0.86 ms at
 // n = 256, back substitution included) and slower when several cells share each CU beside other
 // size classes, so it runs for up to one cell per CU.
 #include <hip/hip_runtime.h>
@@ -416,51 +416,52 @@ __global__ __launch_bounds__ (256, 1) void k_lookahead_f64 (SolveArgs s)
         }
         __syncthreads ();
 
-        // ---- V. the next panel's columns minus panel p's products, one row per thread, into Sn ----
-        if (p + 1 < ns && tid < rows1)
+        // ---- V. the next panel's columns minus panel p's products into Sn: lane (g, h) = (tid >> 2,
+        //      tid & 3) owns four of the 16 columns, with their U12 values in registers, and rows
+        //      g, g + 64, ...; a row's four lanes read 128 contiguous bytes ----
+        if (p + 1 < ns)
         {
           const int nbk1 = min (LA_NB, rows1);
-          const double *row = la_src_row (A, W, ph, first, k1 + tid, n) + k1;
-          double a[LA_NB];
+          const int g = tid >> 2, h = tid & 3;
+          double u[LA_NB][4];
 #pragma unroll
-          for (int c = 0; c < LA_NB; c++)
-            a[c] = c < nbk1 ? row[min (c, nbk1 - 1)] : 0.0;
-          const double *Lr = P + (LA_NB + tid) * LA_PP;
-          // panel p's 16 updates in ascending order (panel p is full: a column follows it)
-#pragma unroll
-          for (int kk = 0; kk < LA_NB; kk += 2)
+          for (int kk = 0; kk < LA_NB; kk++)
           {
-            const double2 l2 = *reinterpret_cast<const double2 *> (Lr + kk);
-#pragma unroll
-            for (int h = 0; h < 2; h++)
-            {
-              const double l = h ? l2.y : l2.x;
-              double u[LA_NB];
-#pragma unroll
-              for (int c = 0; c < LA_NB; c += 2)
-              {
-                const double2 t = *reinterpret_cast<const double2 *> (Us + (kk + h) * LA_UP + c);
-                u[c] = t.x;
-                u[c + 1] = t.y;
-              }
-              double q[LA_NB];
-#pragma unroll
-              for (int c = 0; c < LA_NB; c++)
-              {
-                q[c] = l * u[c];
-                la_hold (q[c]);
-              }
-#pragma unroll
-              for (int c = 0; c < LA_NB; c++)
-                a[c] = a[c] - q[c];
-            }
+            const double2 x = *reinterpret_cast<const double2 *> (Us + kk * LA_UP + 4 * h);
+            const double2 y = *reinterpret_cast<const double2 *> (Us + kk * LA_UP + 4 * h + 2);
+            u[kk][0] = x.x;
+            u[kk][1] = x.y;
+            u[kk][2] = y.x;
+            u[kk][3] = y.y;
           }
-          double *dst = Sn + tid * LA_PP;
+          for (int r = g; r < rows1; r += 64)
+          {
+            const double *row = la_src_row (A, W, ph, first, k1 + r, n) + k1;
+            double a[4];
 #pragma unroll
-          for (int c = 0; c < LA_NB; c += 2)
-            *reinterpret_cast<double2 *> (dst + c) =
-              make_double2 (c < nbk1 ? a[c] : 0.0, c + 1 < nbk1 ? a[c + 1] : 0.0);
-          dst[LA_NB] = wb[k1 + tid];
+            for (int e = 0; e < 4; e++)
+              a[e] = 4 * h + e < nbk1 ? row[min (4 * h + e, nbk1 - 1)] : 0.0;
+            const double *Lr = P + (LA_NB + r) * LA_PP;
+            // panel p's 16 updates in ascending order (panel p is full: a column follows it)
+#pragma unroll
+            for (int kk = 0; kk < LA_NB; kk += 2)
+            {
+              const double2 l2 = *reinterpret_cast<const double2 *> (Lr + kk);
+#pragma unroll
+              for (int e = 0; e < 4; e++)
+                a[e] = a[e] - l2.x * u[kk][e];
+#pragma unroll
+              for (int e = 0; e < 4; e++)
+                a[e] = a[e] - l2.y * u[kk + 1][e];
+            }
+            double *dst = Sn + r * LA_PP + 4 * h;
+            *reinterpret_cast<double2 *> (dst) =
+              make_double2 (4 * h < nbk1 ? a[0] : 0.0, 4 * h + 1 < nbk1 ? a[1] : 0.0);
+            *reinterpret_cast<double2 *> (dst + 2) =
+              make_double2 (4 * h + 2 < nbk1 ? a[2] : 0.0, 4 * h + 3 < nbk1 ? a[3] : 0.0);
+            if (h == 0)
+              Sn[r * LA_PP + LA_NB] = wb[k1 + r];
+          }
         }
         __syncthreads ();
       }
