@@ -199,6 +199,17 @@ extern "C" int hipla_plan_create (hipla_plan *out, int ncells, const int *n, con
     p->groups.push_back (g);
     i = j;
   }
+  // the one-wave kernel for 16 < n <= 32 takes two-wave workgroups by default: 9.6 KB of LDS, so
+  // they fit beside the blocked kernel's workgroups on a CU.  With no class on the blocked kernel
+  // nothing runs beside them, and four-wave workgroups are slightly faster (the autotuner still
+  // measures both).
+  bool has_blocked = false;
+  for (const Group &g : p->groups)
+    has_blocked = has_blocked || g.nmax > wgr_f64_limit ();
+  if (!has_blocked)
+    for (Group &g : p->groups)
+      if (fp64_family_of (g.nmax) == 1)
+        g.variant = 1;
 
   // mixed mode: FP64 routes the fallback re-solve can need
   if (mode == HIPLA_MODE_MIXED)
