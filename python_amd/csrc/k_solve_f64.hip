@@ -86,15 +86,24 @@ __device__ __forceinline__ double sub_oct (double t, const Oct &o)
   return t;
 }
 
+// Eight subtractions of a group after one wait for all four of its reads (the empty asm takes
+// the group as input, so the compiler waits once instead of before each read's first use).
+__device__ __forceinline__ double sub_oct_w (double t, const Oct &o)
+{
+  __asm__ volatile ("" ::"v"(o.a.x), "v"(o.a.y), "v"(o.b.x), "v"(o.b.y), "v"(o.c.x), "v"(o.c.y), "v"(o.d.x),
+                    "v"(o.d.y));
+  return sub_oct (t, o);
+}
+
 // tmp - p[j] - p[j+1] - ... - p[n-1], subtracted in ascending order (a dependent chain).  p is
 // in LDS, 16-byte aligned, and holds +0.0 at every index below j (down to j rounded to even) and
 // from n up to n rounded up to a multiple of eight: the chain runs over whole groups of eight,
 // and subtracting +0.0 leaves every value unchanged (IEEE: t - (+0) == t, -0 and NaN included),
 // so the padding changes nothing.  Three groups are in flight in three register sets that are
-// each reloaded right after their subtractions (no register copies, so no wait on a just-issued
-// read); the reads run up to 40 doubles past the last group (p must be readable up to n + 48).
-// chain_load issues the first three reads, so that a caller can start them early; chain_run
-// subtracts.
+// each reloaded right after their subtractions while later groups remain (no register copies, so
+// no wait on a just-issued read); the last one to five groups issue at most one read past the
+// row.  chain_load issues the first three reads, so that a caller can start them early (p must
+// be readable up to j + 24); chain_run subtracts.
 struct Chain
 {
   Oct c, d, e;
@@ -118,76 +127,101 @@ __device__ __forceinline__ double chain_run (double tmp, const double *p, Chain 
     return tmp;
   const int ng = (n - g + 7) >> 3;   // groups of eight to subtract
   int t = 0;
-  for (; t + 3 <= ng; t += 3, g += 24)
+  for (; t + 6 <= ng; t += 3, g += 24)
   {
     // the scheduling barriers keep each reload right after the subtractions that free its
     // registers (left alone, the scheduler sinks all twelve reads to the end of the iteration,
     // and the next iteration starts by waiting a full LDS round trip)
-    tmp = sub_oct (tmp, h.c);
+    tmp = sub_oct_w (tmp, h.c);
     __builtin_amdgcn_sched_barrier (0);
     h.c = load_oct (p + g + 24);
     __builtin_amdgcn_sched_barrier (0);
-    tmp = sub_oct (tmp, h.d);
+    tmp = sub_oct_w (tmp, h.d);
     __builtin_amdgcn_sched_barrier (0);
     h.d = load_oct (p + g + 32);
     __builtin_amdgcn_sched_barrier (0);
-    tmp = sub_oct (tmp, h.e);
+    tmp = sub_oct_w (tmp, h.e);
     __builtin_amdgcn_sched_barrier (0);
     h.e = load_oct (p + g + 40);
     __builtin_amdgcn_sched_barrier (0);
   }
-  if (t < ng)
-    tmp = sub_oct (tmp, h.c);
-  if (t + 1 < ng)
-    tmp = sub_oct (tmp, h.d);
+  // one to five groups left: groups t .. t+2 are in the registers
+  const int r = ng - t;
+  if (r > 3)
+  {
+    tmp = sub_oct_w (tmp, h.c);
+    __builtin_amdgcn_sched_barrier (0);
+    h.c = load_oct (p + g + 24);
+    __builtin_amdgcn_sched_barrier (0);
+    tmp = sub_oct_w (tmp, h.d);
+    __builtin_amdgcn_sched_barrier (0);
+    h.d = load_oct (p + g + 32);     // past the row when r == 4 (read, never subtracted)
+    __builtin_amdgcn_sched_barrier (0);
+    tmp = sub_oct_w (tmp, h.e);
+    tmp = sub_oct_w (tmp, h.c);
+    if (r > 4)
+      tmp = sub_oct_w (tmp, h.d);
+  }
+  else
+  {
+    tmp = sub_oct_w (tmp, h.c);
+    if (r > 1)
+      tmp = sub_oct_w (tmp, h.d);
+    if (r > 2)
+      tmp = sub_oct_w (tmp, h.e);
+  }
   return tmp;
 }
 
 // Back substitution U x = y in dtrsv order, run by one wave.  Row i of U is at U + i*ldu
-// (columns i..n-1 at their own column index) and y_i at y[i*ys].  Lane l keeps x_{l + 64 m} in
-// xr[m].  Row i subtracts U_ij x_j in ascending j (the chain of dependent FP64 subtractions of
-// GSL's dtrsv, cblas/source_trsv_r.h:35-52) and divides by U_ii.  Its first term U_i,i+1 x_{i+1}
-// needs the x just computed, so it is formed in registers; the products of its later terms only
-// need older x, so the lanes form them in parallel one row ahead (while row i+1's chain runs)
-// and stage them in one of two LDS buffers, where the chain reads them 8 at a time.  Four rows
-// of U are in flight: the row loaded into a slot is used four rows later, so the short chains near
-// the bottom of U do not wait on memory.  xs holds two buffers of XB >= n + 48 doubles; on return
-// xs holds x.
+// (columns i..n-1 at their own column index) and y_i at y[i*ys].  Row i subtracts U_ij x_j in
+// ascending j (the chain of dependent FP64 subtractions of GSL's dtrsv,
+// cblas/source_trsv_r.h:35-52) and divides by U_ii.  Its first term U_i,i+1 x_{i+1} needs the x
+// just computed, so it is formed in registers; the products of its later terms only need older
+// x, so the lanes form them in parallel one row ahead (during row i+1) and stage them in one of
+// two LDS buffers, where the chain reads them 8 at a time; the head of row i's chain is read
+// before row i+1's division, which covers that LDS round trip.  A wave issues about one
+// instruction per four cycles, and an instruction between two subtractions of the chain delays
+// it, so a row keeps its own work small: x lives in LDS (one store per row, two reads per lane),
+// the products are written unmasked and the two places where that leaves a wrong value are
+// overwritten with +0 (column i of row i-1's buffer, whose x is not known yet, and columns n ..
+// n+63, past the matrix), and U comes through D slots (the row's columns, y, U_ii and U_i,i+1),
+// each refilled as soon as its row is done.  xs holds the x buffer and the two product buffers,
+// XB >= max(64 MAXM, n + 64) doubles each; on return the x buffer holds x.
 template <int MAXM>
 __device__ __forceinline__ void wave_backsub (const double *U, size_t ldu, const double *y, size_t ys, int n, int lane,
-                                              double (&xr)[MAXM], double *xs, int XB)
+                                              double *xs, int XB)
 {
-  constexpr int D = 4;
-  double *bufs0 = xs, *bufs1 = xs + XB;
+  constexpr int D = 6;
+  double *xb = xs, *bufs0 = xs + XB, *bufs1 = xs + 2 * XB;
+  // x and both product buffers start at +0 (x_j for j >= n and the padding stay so)
+  for (int k = lane; k < 3 * XB; k += 64)
+    xs[k] = 0.0;
+  // slot sl serves rows n-1-sl, n-1-sl-D, ...; rows n-1 .. n-D+1 are read here, row n-D by the
+  // first row's refill.  A lane past column n-1 reads column n-1 instead (its product lands in
+  // the padding, which is cleared after the products are written).  Row n-1 has no first term:
+  // its U_i,i+1 slot holds +0 and x_n is +0, so y - (+0 * +0) == y.
+  int off[MAXM];
 #pragma unroll
   for (int m = 0; m < MAXM; m++)
-    xr[m] = 0.0;
-  if (lane < 8)
-  {
-    bufs0[n + lane] = 0.0;       // padding of the last group (see chain_run)
-    bufs1[n + lane] = 0.0;
-  }
-  // slots: row r's columns (lane's share), y_r, U_rr and U_r,r+1.  A lane past column n-1
-  // reads column n-1 instead and never uses it: with no lane-dependent branch around the
-  // reads, every refill issues the same number of reads, and the waits for a slot's values
-  // stop at that slot's reads instead of at the last rows'.
+    off[m] = min (lane + 64 * m, n - 1);
   double cu[D][MAXM], cy[D], cd[D], cn[D];
 #pragma unroll
-  for (int sl = 0; sl < D; sl++)
+  for (int sl = 0; sl < D - 1; sl++)
   {
     const int r = uni (max (n - 1 - sl, 0));
     const double *Ur = U + (size_t) r * ldu;
 #pragma unroll
     for (int m = 0; m < MAXM; m++)
-      cu[sl][m] = Ur[min (lane + 64 * m, n - 1)];
+      cu[sl][m] = Ur[off[m]];
     cy[sl] = y[(size_t) r * ys];
-    cd[sl] = U[(size_t) r * ldu + r];
-    cn[sl] = U[(size_t) r * ldu + min (r + 1, n - 1)];
-    // slot by slot, in the order the loop refills them: the waits in the loop are counted for
-    // this order too (the count has to hold on entry as well as around the loop)
+    cd[sl] = Ur[r];
+    cn[sl] = sl == 0 ? 0.0 : Ur[min (r + 1, n - 1)];
     __builtin_amdgcn_sched_barrier (0);
   }
   double xnew = 0.0;             // x_{i+1}, the newest solution value (wave-uniform)
+  __asm__ volatile ("" ::: "memory");
+  Chain h = chain_load (((n - 1) & 1) ? bufs1 : bufs0, n + 1);   // padding only
   for (int i0 = n - 1; i0 >= 0; i0 -= D)
   {
 #pragma unroll
@@ -196,57 +230,49 @@ __device__ __forceinline__ void wave_backsub (const double *U, size_t ldu, const
       const int i = i0 - sl;
       if (i >= 0)
       {
-        // the chain reads other lanes' products: to the compiler those are other threads'
-        // stores, so keep it from reusing anything it read from the buffers before
-        __asm__ volatile ("" ::: "memory");
-        // row i's first products (written while row i+1 ran) are read before anything waits
-        // on x_{i+1}
         const double *pb = (i & 1) ? bufs1 : bufs0;
-        const Chain h = chain_load (pb, i + 2);
-        // row i-1's products of columns i+1 .. n-1 (all x known), +0 at and left of column i and
-        // from column n on (the padding), into the buffer row i-1 reads; row i reads the other
-        // one.  Every lane writes, with no branch (row 0 writes a buffer no row reads).
-        {
-          double *bw = ((i - 1) & 1) ? bufs1 : bufs0;
-#pragma unroll
-          for (int m = 0; m < MAXM; m++)
-          {
-            const int j = lane + 64 * m;
-            bw[j] = (j > i && j < n) ? cu[(sl + 1) % D][m] * xr[m] : 0.0;
-          }
-        }
-        // the first term from registers (+0 for the last row: y - (+0) == y)
-        const double pr = (i + 1 < n) ? cn[sl] * xnew : 0.0;
-        double tmp = cy[sl] - pr;
-        tmp = chain_run (tmp, pb, h, n);
+        double *bw = ((i - 1) & 1) ? bufs1 : bufs0;
+        double tmp = cy[sl] - cn[sl] * xnew;
+        // x_{i+1} into the x buffer (x_n: +0 again) and this lane's x back for row i-1's products
+        xb[i + 1] = xnew;
         __asm__ volatile ("" ::: "memory");
-        xnew = tmp / cd[sl];
+        double xv[MAXM];
 #pragma unroll
         for (int m = 0; m < MAXM; m++)
-          if (lane + 64 * m == i)
-            xr[m] = xnew;
-      }
-      // refill this slot with the row it serves next, unconditionally (row 0 again past the
-      // top): a conditional refill would merge old and new values, and the register copies
-      // that merge needs would wait for the loads just issued
-      const int r = uni (max (i - D, 0));
-      const double *Ur = U + (size_t) r * ldu;
+          xv[m] = xb[lane + 64 * m];
+        // refill row i+1's slot with row i+1-D (row 0 again past the top)
+        {
+          const int sr = (sl + D - 1) % D;
+          const int rr = uni (max (i + 1 - D, 0));
+          const double *Ur = U + (size_t) rr * ldu;
 #pragma unroll
-      for (int m = 0; m < MAXM; m++)
-        cu[sl][m] = Ur[min (lane + 64 * m, n - 1)];
-      cy[sl] = y[(size_t) r * ys];
-      cd[sl] = U[(size_t) r * ldu + r];
-      cn[sl] = U[(size_t) r * ldu + min (r + 1, n - 1)];
-      __builtin_amdgcn_sched_barrier (0);
+          for (int m = 0; m < MAXM; m++)
+            cu[sr][m] = Ur[off[m]];
+          cy[sr] = y[(size_t) rr * ys];
+          cd[sr] = Ur[rr];
+          cn[sr] = Ur[min (rr + 1, n - 1)];
+        }
+        __builtin_amdgcn_sched_barrier (0);
+        tmp = chain_run (tmp, pb, h, n);
+        __builtin_amdgcn_sched_barrier (0);
+        // row i-1's products of columns i+1 .. n-1; column i (x_i not known yet) and columns n
+        // and up are +0.  Row 0 writes a buffer no row reads.
+#pragma unroll
+        for (int m = 0; m < MAXM; m++)
+          bw[lane + 64 * m] = cu[(sl + 1) % D][m] * xv[m];
+        bw[n + lane] = 0.0;
+        bw[i] = 0.0;
+        // row i-1's head: the chain reads other lanes' products, which to the compiler are
+        // other threads' stores
+        __asm__ volatile ("" ::: "memory");
+        h = chain_load (bw, i + 1);
+        __builtin_amdgcn_sched_barrier (0);
+        xnew = tmp / cd[sl];
+      }
     }
   }
-#pragma unroll
-  for (int m = 0; m < MAXM; m++)
-  {
-    const int j = lane + 64 * m;
-    if (j < n)
-      xs[j] = xr[m];
-  }
+  xb[0] = xnew;
+  __asm__ volatile ("" ::: "memory");
 }
 
 // ============================================================================================
@@ -952,8 +978,8 @@ __global__ __launch_bounds__ (64, MAXM <= 4 ? 2 : 1) void k_bsub_f64 (SolveArgs 
 {
   extern __shared__ double smem[];
   constexpr int WPB = 1;                 // one wave per workgroup: the waves spread over every CU
-  constexpr int XB = 64 * MAXM + 48;     // one product buffer (n + 47 readable)
-  constexpr int XP = 2 * XB;             // per wave: two product buffers; x in the first
+  constexpr int XB = 64 * MAXM + 64;     // x, and each product buffer (n + 63 readable)
+  constexpr int XP = 3 * XB;             // per wave: the x buffer and two product buffers
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane (threadIdx.x >> 6);
   double *xs = smem + wv * XP;
@@ -964,13 +990,12 @@ __global__ __launch_bounds__ (64, MAXM <= 4 ? 2 : 1) void k_bsub_f64 (SolveArgs 
     if (s.codes[cell] == 4)
       continue;
     const int n = s.n[cell];
-    double xr[MAXM];
-    wave_backsub<MAXM> (s.F + s.offA[cell], (size_t) n, s.Y + s.offV[cell], 1, n, lane, xr, xs, XB);
+    wave_backsub<MAXM> (s.F + s.offA[cell], (size_t) n, s.Y + s.offV[cell], 1, n, lane, xs, XB);
 
 #pragma unroll
     for (int m = 0; m < MAXM; m++)
       if (lane + 64 * m < n)
-        s.X[s.offV[cell] + lane + 64 * m] = xr[m];
+        s.X[s.offV[cell] + lane + 64 * m] = xs[lane + 64 * m];
 
     // residual check t = A x (gsl_blas_dgemv order: t_r = sum over ascending j of A_rj x_j) and
     // codes 2 / 3 of the first failing row.  Lane l runs the chains of rows l + 64 m side by
@@ -1134,7 +1159,7 @@ template <int MAXM>
 static void launch_bsub_m (const SolveArgs &a, hipStream_t st)
 {
   const int grid = std::max (1, std::min (a.nlist, 8 * hipla_device_info ().num_cu));
-  const size_t sm = 2 * (64 * MAXM + 48) * sizeof (double);
+  const size_t sm = 3 * (64 * MAXM + 64) * sizeof (double);
   hipLaunchKernelGGL ((k_bsub_f64<MAXM>), dim3 (grid), dim3 (64), sm, st, a);
 }
 
