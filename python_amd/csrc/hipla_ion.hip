@@ -386,21 +386,45 @@ struct hipla_ion_s
   hipEvent_t ev_fork = nullptr, ev_join[ION_ROUTES] = { nullptr, nullptr, nullptr, nullptr, nullptr };
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
-  IonArgs gargs{};             // the arguments the graph was captured with
-  bool have_graph = false;
+  // the same iterations once at most one cell per CU is left (the active set only shrinks): the
+  // order 175 < n <= 256 route runs the look-ahead kernel alone
+  hipGraph_t graph_lone = nullptr;
+  hipGraphExec_t gexec_lone = nullptr;
+  IonArgs gargs{};             // the arguments the graphs were captured with
+  bool have_graph = false, have_lone = false;
 };
+
+// The stream (branch) a route's launches go to: the three one-wave routes share one, one after
+// another (short even with every cell active; on streams of their own, one of them would share a
+// hardware queue with the longest route and run before or after it).
+static int ion_branch_of (int r)
+{
+  return r < 3 ? 0 : r;
+}
 
 static int ion_route_of (int n)
 {
   return n <= 16 ? 0 : n <= 32 ? 1 : n <= 64 ? 2 : n <= wgr_f64_limit () ? 3 : 4;
 }
 
-static void ion_free (hipla_ion c)
+static void ion_drop_graphs (hipla_ion c)
 {
   if (c->gexec)
     (void) hipGraphExecDestroy (c->gexec);
   if (c->graph)
     (void) hipGraphDestroy (c->graph);
+  if (c->gexec_lone)
+    (void) hipGraphExecDestroy (c->gexec_lone);
+  if (c->graph_lone)
+    (void) hipGraphDestroy (c->graph_lone);
+  c->gexec = c->gexec_lone = nullptr;
+  c->graph = c->graph_lone = nullptr;
+  c->have_graph = c->have_lone = false;
+}
+
+static void ion_free (hipla_ion c)
+{
+  ion_drop_graphs (c);
   void *bufs[] = { c->dA, c->dB, c->dX, c->dcodes, c->d_xne, c->d_ed, c->d_newden, c->d_iter, c->d_active, c->d_route,
                    c->d_lists, c->d_nactive };
   for (void *b : bufs)
@@ -492,13 +516,16 @@ extern "C" int hipla_ion_create (hipla_ion *out, int ncells, const int *n, const
   ICHECK (hipEventCreateWithFlags (&c->ev_fork, hipEventDisableTiming));
   hipla_count_resource ();
   for (int r = 0; r < ION_ROUTES; r++)
-    if (c->route_used[r])
+  {
+    const int br = ion_branch_of (r);
+    if (c->route_used[r] && !c->rs[br])
     {
-      ICHECK (hipStreamCreateWithFlags (&c->rs[r], hipStreamNonBlocking));
-      ICHECK (hipEventCreateWithFlags (&c->ev_join[r], hipEventDisableTiming));
+      ICHECK (hipStreamCreateWithFlags (&c->rs[br], hipStreamNonBlocking));
+      ICHECK (hipEventCreateWithFlags (&c->ev_join[br], hipEventDisableTiming));
       hipla_count_resource ();
       hipla_count_resource ();
     }
+  }
 #undef ICHECK
   *out = c;
   return 0;
@@ -549,8 +576,9 @@ static size_t ion_asm_smem (int nmax)
   return (size_t) nmax * ION_NCOL * sizeof (double) + 2 * (size_t) nmax * sizeof (int);
 }
 
-// one iteration: reset, assemble, the FP64 solves of the listed cells, update
-static int ion_enqueue_iteration (hipla_ion c, const IonArgs &a, hipStream_t st)
+// one iteration: reset, assemble, the FP64 solves of the listed cells, update.  lone: at most
+// lookahead_f64_max_cells () cells are active.
+static int ion_enqueue_iteration (hipla_ion c, const IonArgs &a, hipStream_t st, bool lone)
 {
   const DeviceInfo &dev = hipla_device_info ();
   const int nmax = *std::max_element (c->n.begin (), c->n.end ());
@@ -560,12 +588,17 @@ static int ion_enqueue_iteration (hipla_ion c, const IonArgs &a, hipStream_t st)
   hipLaunchKernelGGL (k_ion_assemble, dim3 (agrid), dim3 (ION_ASM_THREADS), ion_asm_smem (nmax), st, a);
   HIPLA_CHECK (hipGetLastError ());
   HIPLA_CHECK (hipEventRecord (c->ev_fork, st));
-  for (int r = 0; r < ION_ROUTES; r++)
+  // three branches (ion_branch_of), largest orders first
+  bool forked[ION_ROUTES] = { false, false, false, false, false };
+  for (int r = ION_ROUTES - 1; r >= 0; r--)
   {
     if (!c->route_used[r])
       continue;
-    hipStream_t rst = c->rs[r];
-    HIPLA_CHECK (hipStreamWaitEvent (rst, c->ev_fork, 0));
+    const int br = ion_branch_of (r);
+    hipStream_t rst = c->rs[br];
+    if (!forked[br])
+      HIPLA_CHECK (hipStreamWaitEvent (rst, c->ev_fork, 0));
+    forked[br] = true;
     SolveArgs sa;
     sa.n = a.n;
     sa.offA = a.offA;
@@ -585,12 +618,18 @@ static int ion_enqueue_iteration (hipla_ion c, const IonArgs &a, hipStream_t st)
       HIPLA_CHECK (launch_wave_f64 (sa, r == 0 ? 16 : r == 1 ? 32 : 64, false, 2 * dev.num_cu, rst));
     else if (r == 3)
       HIPLA_CHECK (launch_wgr_f64 (sa, c->route_nmax[3], false, dev.num_cu, rst));
+    else if (lone && c->route_nmax[4] <= LA_ROUTE_MAXN)
+      HIPLA_CHECK (launch_lookahead_route_f64 (sa, c->route_nmax[4], false, rst));
     else
       HIPLA_CHECK (launch_blocked_f64 (sa, c->route_nmax[4], blocked_f64_wgs_per_cu (c->route_nmax[4]) * dev.num_cu,
                                        false, rst));
-    HIPLA_CHECK (hipEventRecord (c->ev_join[r], rst));
-    HIPLA_CHECK (hipStreamWaitEvent (st, c->ev_join[r], 0));
   }
+  for (int br = 0; br < ION_ROUTES; br++)
+    if (forked[br])
+    {
+      HIPLA_CHECK (hipEventRecord (c->ev_join[br], c->rs[br]));
+      HIPLA_CHECK (hipStreamWaitEvent (st, c->ev_join[br], 0));
+    }
   const int ugrid = std::max (1, std::min (c->ncells, 8 * dev.num_cu));
   hipLaunchKernelGGL (k_ion_update, dim3 (ugrid), dim3 (64), 0, st, a);
   HIPLA_CHECK (hipGetLastError ());
@@ -608,6 +647,27 @@ static bool same_args (const IonArgs &x, const IonArgs &y)
          x.iters == y.iters && x.codes_out == y.codes_out;
 }
 
+// capture ION_CHUNK iterations into a graph and instantiate it
+static int ion_capture (hipla_ion c, const IonArgs &a, bool lone, hipGraph_t *graph, hipGraphExec_t *gexec)
+{
+  HIPLA_CHECK (hipStreamBeginCapture (c->cap, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  for (int k = 0; k < ION_CHUNK && rc == 0; k++)
+    rc = ion_enqueue_iteration (c, a, c->cap, lone);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture (c->cap, &g);
+  if (rc)
+  {
+    if (g)
+      (void) hipGraphDestroy (g);
+    return rc;
+  }
+  HIPLA_CHECK (e);
+  *graph = g;
+  HIPLA_CHECK (hipGraphInstantiate (gexec, g, nullptr, nullptr, 0));
+  return 0;
+}
+
 extern "C" int hipla_ion_populations (hipla_ion c, const hipla_ion_rates *rates, double *dDensity, double *dNe,
                                       int *dStatus, int *dIters, int *dCodes, void *stream)
 {
@@ -623,43 +683,32 @@ extern "C" int hipla_ion_populations (hipla_ion c, const hipla_ion_rates *rates,
   const DeviceInfo &dev = hipla_device_info ();
   hipLaunchKernelGGL (k_ion_init, dim3 (std::max (1, std::min (c->ncells, 8 * dev.num_cu))), dim3 (64), 0, st, a);
   HIPLA_CHECK (hipGetLastError ());
-  // ION_CHUNK iterations per graph replay, captured once per argument set
-  if (!c->have_graph || !same_args (a, c->gargs))
-  {
-    if (c->gexec)
-      (void) hipGraphExecDestroy (c->gexec);
-    if (c->graph)
-      (void) hipGraphDestroy (c->graph);
-    c->gexec = nullptr;
-    c->graph = nullptr;
-    c->have_graph = false;
-    HIPLA_CHECK (hipStreamBeginCapture (c->cap, hipStreamCaptureModeThreadLocal));
-    int rc = 0;
-    for (int k = 0; k < ION_CHUNK && rc == 0; k++)
-      rc = ion_enqueue_iteration (c, a, c->cap);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture (c->cap, &g);
-    if (rc)
-    {
-      if (g)
-        (void) hipGraphDestroy (g);
-      return rc;
-    }
-    HIPLA_CHECK (e);
-    c->graph = g;
-    HIPLA_CHECK (hipGraphInstantiate (&c->gexec, g, nullptr, nullptr, 0));
-    c->gargs = a;
-    c->have_graph = true;
-  }
+  // ION_CHUNK iterations per graph replay, captured once per argument set (the lone variant when
+  // first needed)
+  if (!same_args (a, c->gargs))
+    ion_drop_graphs (c);
+  c->gargs = a;
+  bool lone = false;
   // at most MAXITERATIONS solves per cell: every cell has left after that many iterations
   const int max_replays = (ION_MAXITERATIONS + ION_CHUNK - 1) / ION_CHUNK + 1;
   for (int k = 0; k < max_replays; k++)
   {
-    HIPLA_CHECK (hipGraphLaunch (c->gexec, st));
+    bool &have = lone ? c->have_lone : c->have_graph;
+    if (!have)
+    {
+      int rc = ion_capture (c, a, lone, lone ? &c->graph_lone : &c->graph, lone ? &c->gexec_lone : &c->gexec);
+      if (rc)
+        return rc;
+      have = true;
+    }
+    HIPLA_CHECK (hipGraphLaunch (lone ? c->gexec_lone : c->gexec, st));
     HIPLA_CHECK (hipMemcpyAsync (c->h_nactive, c->d_nactive, sizeof (int), hipMemcpyDeviceToHost, st));
     HIPLA_CHECK (hipStreamSynchronize (st));
     if (*c->h_nactive == 0)
       return 0;
+    // cells only leave the iteration, so from here on every list holds at most this many
+    if (*c->h_nactive <= lookahead_f64_max_cells ())
+      lone = true;
   }
   hipla_set_detail ("hipla_ion_populations: %d cells still active after %d iterations", *c->h_nactive,
                     max_replays * ION_CHUNK);

@@ -21,6 +21,9 @@ hipError_t launch_wave_f64 (const SolveArgs &a, int nc, bool keep, int grid, hip
 hipError_t launch_wgr_f64 (const SolveArgs &a, int nmax, bool keep, int grid, hipStream_t st);
 hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, bool keep, hipStream_t st,
                                bool with_bsub = true);
+// the look-ahead route alone (lists known to hold at most lookahead_f64_max_cells () cells)
+#define LA_ROUTE_MAXN 256
+hipError_t launch_lookahead_route_f64 (const SolveArgs &a, int nmax, bool keep, hipStream_t st);
 // back substitution + residual check of cells a workgroup factor kernel has factored
 hipError_t launch_bsub_cells (const SolveArgs &a, int nmax, hipStream_t st);
 int blocked_f64_wgs_per_cu (int nmax);   // resident workgroups per CU (persistent grid size)

@@ -1237,6 +1237,26 @@ hipError_t launch_blocked_f64 (const SolveArgs &a, int nmax, int grid, bool keep
   return launch_bsub_f64 (a, nmax, st);
 }
 
+// The order 175 < n <= 256 route for a device-side list its caller knows to hold at most one cell
+// per CU (lookahead_f64_max_cells): the look-ahead factor kernel alone, then the back
+// substitution, with no launch of the blocked kernel standing by for longer lists.  Correct for
+// any list length (the kernel loops over its list), only slower than the blocked kernel past that
+// bound.
+hipError_t launch_lookahead_route_f64 (const SolveArgs &a, int nmax, bool keep, hipStream_t st)
+{
+  if (a.nlist <= 0)
+    return hipSuccess;
+  if (nmax > LA_ROUTE_MAXN || !a.F || !a.Y)
+    return hipErrorInvalidValue;
+  SolveArgs w = a;
+  w.cnt_lo = 0;
+  w.cnt_hi = 0x7fffffff;
+  hipError_t e = launch_lookahead_f64 (w, std::min (lookahead_f64_max_cells (), a.nlist), keep, st);
+  if (e != hipSuccess)
+    return e;
+  return launch_bsub_f64 (w, nmax, st);
+}
+
 // Name of the FP64 kernel instantiation a group of largest order nmax runs (as profilers print
 // it), for the plan description.
 void fp64_kernel_symbol (int nmax, bool keep, char *buf, size_t len, int count)
