@@ -38,6 +38,7 @@
 #define ION_CXI (1 << 20)
 #define ION_MAXJUMP 10
 #define ION_NCOL (ION_MAXJUMP + 2)   // per column: (j-1, j), (j, j), (j+1 .. j+10, j)
+#define ION_RFIRST (1 << 30)         // assembly's row flags: an element's abundance row
 #define ION_DENSITY_MIN 1.e-20       // source/atomic.h:9
 #define ION_MAXITERATIONS 200        // source/python.h:1211
 #define ION_FRACTIONAL_ERROR 0.03    // source/python.h:1212
@@ -137,8 +138,11 @@ __global__ __launch_bounds__ (ION_ASM_THREADS) void k_ion_assemble (IonArgs s)
       continue;
     const int n = s.n[cell];
     const int64_t o = s.offV[cell];
-    double *colv = smem;                                        // [n][ION_NCOL]
-    int *el = reinterpret_cast<int *> (smem + (size_t) n * ION_NCOL);
+    // per row: element | ION_RFIRST on abundance rows, padded to whole groups of 16 rows (first:
+    // 16-byte aligned for the grouped reads); then the columns' rates, elements and flags
+    int *rinfo = reinterpret_cast<int *> (smem);
+    double *colv = smem + ((n + 15) & ~15) / 2;                 // [n][ION_NCOL]
+    int *el = reinterpret_cast<int *> (colv + (size_t) n * ION_NCOL);
     int *bt = el + n;
     if (tid == 0)
     {
@@ -149,8 +153,10 @@ __global__ __launch_bounds__ (ION_ASM_THREADS) void k_ion_assemble (IonArgs s)
     for (int i = tid; i < n; i += ION_ASM_THREADS)
     {
       const int b = s.r.bits[o + i];
-      el[i] = s.r.elem[o + i];
+      const int e = s.r.elem[o + i];
+      el[i] = e;
       bt[i] = b;
+      rinfo[i] = e | ((b & ION_FIRST) ? ION_RFIRST : 0);
       // the reference's loop over ions keeps the last H I / H II it meets (:231-241)
       if (ION_Z (b) == 1 && ION_ISTATE (b) == 1)
         atomicMax (&s_h1, i);
@@ -229,24 +235,43 @@ __global__ __launch_bounds__ (ION_ASM_THREADS) void k_ion_assemble (IonArgs s)
       }
     }
     __syncthreads ();
-    // write A row by row and b; the first row of each element is its abundance constraint
+    // write A and b; the first row of each element is its abundance constraint.  A thread writes
+    // its column j: every row gets 0, or 1 on an abundance row of j's element (the row flags read
+    // 16 rows at a time, one LDS round trip per 16 stores); then the rows j-1 .. j+ION_MAXJUMP
+    // that are not abundance rows get the column's rates (a later store of the same thread to the
+    // same address, which wins).  Each row's stores are coalesced across the workgroup.
     double *A = s.A + s.offA[cell];
-    for (int i = 0; i < n; i++)
+    for (int j = tid; j < n; j += ION_ASM_THREADS)
     {
-      const bool first = (bt[i] & ION_FIRST) != 0;
-      const int ei = el[i];
-      double *row = A + (int64_t) i * n;
-      for (int j = tid; j < n; j += ION_ASM_THREADS)
+      const int ej = el[j];
+      double cvj[ION_NCOL];
+#pragma unroll
+      for (int k = 0; k < ION_NCOL; k++)
+        cvj[k] = colv[(size_t) j * ION_NCOL + k];
+      double *Aj = A + j;
+      for (int i0 = 0; i0 < n; i0 += 16)
       {
-        double v;
-        if (first)
-          v = el[j] == ei ? 1.0 : 0.0;
-        else
+        int ri[16];
+#pragma unroll
+        for (int q = 0; q < 16; q += 4)
         {
-          const int d = i - j;
-          v = (d >= -1 && d <= ION_MAXJUMP) ? colv[(size_t) j * ION_NCOL + 1 + d] : 0.0;
+          const int4 t = *reinterpret_cast<const int4 *> (rinfo + i0 + q);
+          ri[q] = t.x;
+          ri[q + 1] = t.y;
+          ri[q + 2] = t.z;
+          ri[q + 3] = t.w;
         }
-        row[j] = v;
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+          if (i0 + q < n)
+            Aj[(int64_t) (i0 + q) * n] = (ri[q] == (ej | ION_RFIRST)) ? 1.0 : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < ION_NCOL; k++)
+      {
+        const int i = j + k - 1;
+        if (i >= 0 && i < n && !(rinfo[i] & ION_RFIRST))
+          Aj[(int64_t) i * n] = cvj[k];
       }
     }
     for (int i = tid; i < n; i += ION_ASM_THREADS)
@@ -573,7 +598,9 @@ static IonArgs ion_args (hipla_ion c, const hipla_ion_rates *r, double *dDensity
 
 static size_t ion_asm_smem (int nmax)
 {
-  return (size_t) nmax * ION_NCOL * sizeof (double) + 2 * (size_t) nmax * sizeof (int);
+  // rinfo padded to whole groups of 16 rows, colv, el, bt
+  return (size_t) ((nmax + 15) & ~15) * sizeof (int) + (size_t) nmax * ION_NCOL * sizeof (double) +
+         2 * (size_t) nmax * sizeof (int);
 }
 
 // one iteration: reset, assemble, the FP64 solves of the listed cells, update.  lone: at most

@@ -38,6 +38,19 @@ def test_uneven_batch_equals_oracle(hip, oracle_lib):
     it.close()
 
 
+def test_odd_orders_equal_oracle(hip, oracle_lib):
+    """Odd orders (13 .. 255): the assembly's row flags and rates in LDS for matrices whose order is
+    not a multiple of the grouped reads, through every route, bit for bit."""
+    from python_amd.generators import ion_iteration_batch
+    from python_amd.ion import IonIteration
+    batch = ion_iteration_batch(scale=0.02, seed=3, sizes=(13, 31, 65, 97, 129, 201, 255))
+    want = oracle_lib.ion_populations_batch(batch, nthreads=8)
+    print("cells", batch.ncells, "orders", sorted(set(np.asarray(batch.n).tolist())), flush=True)
+    it = IonIteration(batch)
+    _compare(batch, it.populations(), want)
+    it.close()
+
+
 def test_restart_from_converged_and_singular_cells(hip, oracle_lib):
     """Restarting from the converged densities (most cells stop at once), and cells with no
     process into or out of H II (its row is zero: solve code 4, return -1, densities kept)."""
