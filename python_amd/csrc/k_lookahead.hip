@@ -13,7 +13,8 @@
 //    columns and b) in registers.  A pivot step is a choice among the lane's four rows, a wave
 //    reduction and readlanes of the pivot row into scalar registers: no barrier.
 //  * Look-ahead: while wave (p+1) mod 4 factors panel p+1, the other three waves apply panel p to
-//    the columns right of it; the panel chain overlaps that trailing update.
+//    the columns right of it; the panel chain overlaps that trailing update, and the panel wave
+//    takes its share of the update once panel p+1 is out.
 //  * The kernel runs for up to one cell per CU, so a workgroup has the CU to itself: 140 KB of LDS
 //    hold two panel buffers (the panel being applied and the one being factored), the U12 rows
 //    and the next panel's updated rows.
@@ -30,12 +31,12 @@
 //      the columns of panel p+1 and put the rows, with b, in LDS (Sn);
 //   T. wave (p+1) mod 4 factors panel p+1 from Sn and writes it out (the other panel buffer, the
 //      factor store, b, its position map, GSL's ipiv); the other waves apply A22 -= L21 U12 to
-//      the columns from k1 + 16 on.
+//      the columns from k1 + 16 on, joined by the panel wave when it is done.
 // The first panel reads A and b directly; a row of A that no pivoting moved is read from A until
 // the first trailing update has written it to the factor store.
 //
 // Against k_blocked_f64<256> (four-wave panel steps about 2 400 cycles apart, two workgroups per
-// CU filling each other's gaps), this kernel is faster for one cell (0.67 against 0.86 ms at
+// CU filling each other's gaps), this kernel is faster for one cell (0.61 against 0.86 ms at
 // n = 256, back substitution included) and slower when several cells share each CU beside other
 // size classes, so it runs for up to one cell per CU.
 #include <hip/hip_runtime.h>
@@ -549,7 +550,8 @@ __global__ __launch_bounds__ (256, 1) void k_lookahead_f64 (SolveArgs s)
           if (KEEP && s.piv && lane < nbk1)
             s.piv[s.offV[cell] + k1 + lane] = k1 + s_ipiv[lane];
         }
-        else if (p >= 0)
+        // the panel wave, once panel p+1 is out, takes strips too
+        if (p >= 0)
         {
           const int nunit = ns - p - 2;
           for (;;)
