@@ -11,7 +11,8 @@
 //
 //  * A panel is factored by ONE wave: lane l holds the panel rows l, l+64, l+128, l+192 (16
 //    columns and b) in registers.  A pivot step is a choice among the lane's four rows, a wave
-//    reduction and readlanes of the pivot row into scalar registers: no barrier.
+//    reduction, the pivot read into scalar registers and the rest of the pivot row broadcast
+//    through the LDS crossbar (ds_bpermute): no barrier.
 //  * Look-ahead: while wave (p+1) mod 4 factors panel p+1, the other three waves apply panel p to
 //    the columns right of it; the panel chain overlaps that trailing update, and the panel wave
 //    takes its share of the update once panel p+1 is out.
@@ -76,36 +77,45 @@ struct LaPanel
 };
 
 // Row slot sb (wave-uniform) of lane wl, columns KK .. 16, to every lane (scalar registers).
+// lane src's double in every lane, through the LDS crossbar (no LDS memory): the results land in
+// vector registers, and one instruction per dword moves a whole row's value
+__device__ __forceinline__ double la_bcast (double v, int src)
+{
+  const unsigned long long u = (unsigned long long) __double_as_longlong (v);
+  const int lo = __builtin_amdgcn_ds_bpermute (src << 2, (int) (unsigned) u);
+  const int hi = __builtin_amdgcn_ds_bpermute (src << 2, (int) (unsigned) (u >> 32));
+  return __longlong_as_double ((long long) (((unsigned long long) (unsigned) hi << 32) | (unsigned) lo));
+}
+
+// the pivot row's columns KK+1 .. 16 (b last) from lane wl's slot sb, in every lane, and its
+// pivot U_KK,KK in scalar registers (the multiplier rule branches on it)
 template <int KK>
-__device__ __forceinline__ void la_pivot_row (const LaPanel &pn, int sb, int wl, double (&uu)[LA_NB + 1])
+__device__ __forceinline__ double la_pivot_row (const LaPanel &pn, int sb, int wl, double (&uu)[LA_NB + 1])
 {
   switch (sb)
   {
   case 0:
 #pragma unroll
-    for (int c = KK; c <= LA_NB; c++)
-      uu[c] = rl_f64 (pn.v[0][c], wl);
-    break;
+    for (int c = KK + 1; c <= LA_NB; c++)
+      uu[c] = la_bcast (pn.v[0][c], wl);
+    return rl_f64 (pn.v[0][KK], wl);
   case 1:
 #pragma unroll
-    for (int c = KK; c <= LA_NB; c++)
-      uu[c] = rl_f64 (pn.v[1][c], wl);
-    break;
+    for (int c = KK + 1; c <= LA_NB; c++)
+      uu[c] = la_bcast (pn.v[1][c], wl);
+    return rl_f64 (pn.v[1][KK], wl);
   case 2:
 #pragma unroll
-    for (int c = KK; c <= LA_NB; c++)
-      uu[c] = rl_f64 (pn.v[2][c], wl);
-    break;
+    for (int c = KK + 1; c <= LA_NB; c++)
+      uu[c] = la_bcast (pn.v[2][c], wl);
+    return rl_f64 (pn.v[2][KK], wl);
   default:
 #pragma unroll
-    for (int c = KK; c <= LA_NB; c++)
-      uu[c] = rl_f64 (pn.v[3][c], wl);
-    break;
+    for (int c = KK + 1; c <= LA_NB; c++)
+      uu[c] = la_bcast (pn.v[3][c], wl);
+    return rl_f64 (pn.v[3][KK], wl);
   }
 }
-
-// One elimination step KK of the held panel (GSL's idamax choice by position, its swap as a
-// position exchange, its multiplier rule, the rank-1 update of the panel's columns and b).
 template <int KK>
 __device__ __forceinline__ void la_step (LaPanel &pn, int lane, int *ipiv, int &singular)
 {
@@ -127,8 +137,7 @@ __device__ __forceinline__ void la_step (LaPanel &pn, int lane, int *ipiv, int &
   const int pq = __builtin_amdgcn_readlane (bp, wl);
   const int sb = __builtin_amdgcn_readlane (bs, wl);
   double uu[LA_NB + 1];
-  la_pivot_row<KK> (pn, sb, wl, uu);
-  const double piv = uu[KK];
+  const double piv = la_pivot_row<KK> (pn, sb, wl, uu);
   singular |= (piv == 0.0);
   double l[LA_MR];
   if (fabs (piv) >= DBL_MIN)
