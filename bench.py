@@ -132,7 +132,7 @@ def _emit(out):
 
 def bench_c1(args):
     """The plain small case: 900 cells of the n = 30 ionization matrix (H, He, C, N, O + slack,
-    block-tridiagonal), seed 1.  The batched FP64 GSL-order solve (one graph launch per step)
+    block-tridiagonal), seed 1.  The batched FP64 GSL-order solve (one kernel launch per step)
     against one launch per matrix and the CPU oracle on the same host, values and codes checked
     against the oracle in every cell."""
     import torch
@@ -178,7 +178,7 @@ def bench_c1(args):
                                   "ms_per_step": round(ms_per, 3)},
         "roofline": {"bound": "hbm", "achieved": round(bw, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(bw / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": plan.describe()["groups"][0]["fp64_kernel"] + " (one graph launch)",
+                     "kernel": plan.describe()["groups"][0]["fp64_kernel"] + " (one launch per step)",
                      "algorithmic_bytes_per_launch": round(byts), "algorithmic_flops_per_launch": round(flops)},
         "cpu_baseline": {"value": round(reps_c * batch.ncells / total, 1), "unit": "solves/s", "cores": cores,
                          "kind": "port", "sample": f"the 900-cell batch x {reps_c} ({total:.1f} s, one OpenMP "
