@@ -199,7 +199,7 @@ int hipla_batch_read (const char *path, int *n, int64_t *offA, int64_t *offV, do
 #define HIPLA_ION_MAXJUMP 10          /* inner-shell electron yields per record (prob[0..9])      */
 
 /* Device pointers; per-ion arrays in B's layout (cell c's ions at offV[c]).  elem is an element
- * id that is unique within a cell (the reference compares ion[].z).  Inner-shell record k of
+ * id in 0 .. 2^30 - 1 that is unique within a cell (the reference compares ion[].z).  Inner-shell record k of
  * ion i (inner_beg[i] <= k < inner_beg[i] + inner_cnt[i], global record indices) ionizes i with
  * rate inner_rate[k] into the stages i + d, d = 1 .. min(z - istate + 1, 11) - 1, with yields
  * inner_yield[10 k + d - 1].  has_cx: n_charge_exchange > 0 (charge-exchange recombination by

@@ -156,7 +156,9 @@ __global__ __launch_bounds__ (ION_ASM_THREADS) void k_ion_assemble (IonArgs s)
       const int e = s.r.elem[o + i];
       el[i] = e;
       bt[i] = b;
-      rinfo[i] = e | ((b & ION_FIRST) ? ION_RFIRST : 0);
+      // the flag bit set on abundance rows only (cleared otherwise, so no other row can match
+      // an element id below 2^30 with the flag)
+      rinfo[i] = (b & ION_FIRST) ? (e | ION_RFIRST) : (e & ~ION_RFIRST);
       // the reference's loop over ions keeps the last H I / H II it meets (:231-241)
       if (ION_Z (b) == 1 && ION_ISTATE (b) == 1)
         atomicMax (&s_h1, i);
